@@ -1,0 +1,86 @@
+/*
+ * magma_dbatched.h -- fixed-size batched double-precision factor / solve /
+ * GEMM entry points.  Each prototype is the reference's z-precision prototype
+ * (include/magma_zbatched.h, cited per function) with magmaDoubleComplex ->
+ * double, i.e. what tools/codegen.py generates as magma_dbatched.h.
+ *
+ * Conventions kept from the reference: all arrays device-resident (including
+ * the pointer arrays); column-major matrices; ipiv 1-based; info_array zeroed
+ * by the callee, info[i] > 0 a numerical failure of matrix i; return 0 or
+ * -(position of the first bad argument) after magma_xerbla, or a MAGMA_ERR_*
+ * code.  Every routine is enqueued on `queue` and the factor/solve routines
+ * synchronize the queue before returning, as the reference's drivers do
+ * (src/zgetrf_batched.cpp:354, src/zgetrs_batched.cpp:261,
+ * src/zpotrf_batched.cpp:338, src/zpotrs_batched.cpp:237).
+ */
+#ifndef MAGMA_AMD_DBATCHED_H
+#define MAGMA_AMD_DBATCHED_H
+
+#include "magma_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* include/magma_zbatched.h:464-470 -- LU with partial pivoting, A = P*L*U */
+magma_int_t magma_dgetrf_batched(magma_int_t m, magma_int_t n, double **dA_array, magma_int_t lda,
+                                 magma_int_t **ipiv_array, magma_int_t *info_array,
+                                 magma_int_t batchCount, magma_queue_t queue);
+
+/* include/magma_zbatched.h:395-401 -- solve op(A) X = B with the getrf factors */
+magma_int_t magma_dgetrs_batched(magma_trans_t trans, magma_int_t n, magma_int_t nrhs,
+                                 double **dA_array, magma_int_t ldda, magma_int_t **dipiv_array,
+                                 double **dB_array, magma_int_t lddb,
+                                 magma_int_t batchCount, magma_queue_t queue);
+
+/* include/magma_zbatched.h:617-623 -- getrf + getrs */
+magma_int_t magma_dgesv_batched(magma_int_t n, magma_int_t nrhs, double **dA_array, magma_int_t ldda,
+                                magma_int_t **dipiv_array, double **dB_array, magma_int_t lddb,
+                                magma_int_t *dinfo_array, magma_int_t batchCount, magma_queue_t queue);
+
+/* include/magma_zbatched.h:324-328 -- Cholesky A = L*L' (MagmaLower; MagmaUpper
+ * is rejected with -1 exactly as the reference's batched kernels do,
+ * magmablas/zpotf2_kernels.cu:153-156) */
+magma_int_t magma_dpotrf_batched(magma_uplo_t uplo, magma_int_t n, double **dA_array, magma_int_t lda,
+                                 magma_int_t *info_array, magma_int_t batchCount, magma_queue_t queue);
+
+/* include/magma_zbatched.h:380-385 -- solve with the potrf factor (Lower or Upper) */
+magma_int_t magma_dpotrs_batched(magma_uplo_t uplo, magma_int_t n, magma_int_t nrhs,
+                                 double **dA_array, magma_int_t ldda, double **dB_array, magma_int_t lddb,
+                                 magma_int_t batchCount, magma_queue_t queue);
+
+/* include/magma_zbatched.h:387-393 -- potrf + potrs */
+magma_int_t magma_dposv_batched(magma_uplo_t uplo, magma_int_t n, magma_int_t nrhs,
+                                double **dA_array, magma_int_t ldda, double **dB_array, magma_int_t lddb,
+                                magma_int_t *dinfo_array, magma_int_t batchCount, magma_queue_t queue);
+
+/* include/magma_zbatched.h:88-108 -- C = alpha op(A) op(B) + beta C, fixed sizes */
+void magma_dgemm_batched(magma_trans_t transA, magma_trans_t transB,
+                         magma_int_t m, magma_int_t n, magma_int_t k, double alpha,
+                         double const *const *dA_array, magma_int_t ldda,
+                         double const *const *dB_array, magma_int_t lddb, double beta,
+                         double **dC_array, magma_int_t lddc, magma_int_t batchCount, magma_queue_t queue);
+void magmablas_dgemm_batched(magma_trans_t transA, magma_trans_t transB,
+                             magma_int_t m, magma_int_t n, magma_int_t k, double alpha,
+                             double const *const *dA_array, magma_int_t ldda,
+                             double const *const *dB_array, magma_int_t lddb, double beta,
+                             double **dC_array, magma_int_t lddc, magma_int_t batchCount, magma_queue_t queue);
+
+/* include/magma_zbatched.h:28-41 -- pointer-array helpers
+ * output_array[i] = input + row + column*lda + i*batch_offset */
+void magma_dset_pointer(double **output_array, double *input, magma_int_t lda,
+                        magma_int_t row, magma_int_t column, magma_int_t batch_offset,
+                        magma_int_t batchCount, magma_queue_t queue);
+/* output_array[i] = input_array[i] + row + column*lda */
+void magma_ddisplace_pointers(double **output_array, double **input_array, magma_int_t lda,
+                              magma_int_t row, magma_int_t column,
+                              magma_int_t batchCount, magma_queue_t queue);
+/* include/magma_batched.h:47-53 -- output_array[i] = input + row + column*lda + i*batchSize */
+void magma_iset_pointer(magma_int_t **output_array, magma_int_t *input, magma_int_t lda,
+                        magma_int_t row, magma_int_t column, magma_int_t batchSize,
+                        magma_int_t batchCount, magma_queue_t queue);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,168 @@
+// gemm.hip -- batched and vbatched DGEMM, C = alpha op(A) op(B) + beta C, on
+// FP64 MFMA (v_mfma_f64_16x16x4f64).
+//
+// Each wavefront owns one 32x32 tile of C (2x2 MFMA blocks of 16x16) and
+// walks k four at a time; a workgroup holds 4 independent wavefronts.  The
+// grid is persistent: wavefronts stride over a flat tile list.  For vbatched
+// calls the list is the device prefix sum of per-matrix tile counts
+// (workqueue.hip), so a matrix contributes exactly its own tiles instead of a
+// grid sized by the largest matrix with most blocks exiting
+// (magmablas/gemm_template_kernel_vbatched.cuh:18-304,
+// magmablas/dgemm_vbatched_core.cu:27-266).
+//
+// MFMA operand layout (16x16x4, wave64): lane l supplies A(l%16, l/16) and
+// B(l/16, l%16) of the 16x4 / 4x16 blocks and receives D(4*(l/16)+i, l%16),
+// i = 0..3, of the 16x16 result.
+#include "kernels.h"
+#include "workqueue.h"
+
+namespace magma_amd {
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int kWaves = 4;
+constexpr int kTile = 32;
+
+__device__ __forceinline__ int pick(const int *arr, int fixed, int id) { return arr ? arr[id] : fixed; }
+
+template <bool TA, bool TB>
+__global__ void __launch_bounds__(64 * kWaves) gemm_mfma_kernel(GemmSet g, const int *prefix, int fixed_tiles)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int li = lane & 15;          // row within a 16-block (A) / column (B, D)
+    const int lk = lane >> 4;          // k within the 4-slice (A, B) / row quad (D)
+    const int total = prefix ? prefix[g.count] : fixed_tiles * g.count;
+    const int stride = gridDim.x * kWaves;
+
+    for (int t = blockIdx.x * kWaves + wave; t < total; t += stride) {
+        int s, local;
+        if (prefix) {
+            int lo = 0, hi = g.count;          // largest s with prefix[s] <= t
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (prefix[mid] <= t) lo = mid; else hi = mid;
+            }
+            s = lo;
+            local = t - prefix[s];
+        } else {
+            s = t / fixed_tiles;
+            local = t - s * fixed_tiles;
+        }
+        const int id = g.index ? g.index[s] : s;
+        const int m = pick(g.m_arr, g.m, id), n = pick(g.n_arr, g.n, id), k = pick(g.k_arr, g.k, id);
+        const int lda = pick(g.lda_arr, g.lda, id), ldb = pick(g.ldb_arr, g.ldb, id);
+        const int ldc = pick(g.ldc_arr, g.ldc, id);
+        const int mt = (m + kTile - 1) / kTile;
+        const int row0 = (local % mt) * kTile;
+        const int col0 = (local / mt) * kTile;
+        const double *A = g.A[id];
+        const double *B = g.B[id];
+        double *C = g.C[id];
+
+        d4 acc[2][2];
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = d4{0.0, 0.0, 0.0, 0.0};
+
+        const int r0 = row0 + li, r1 = row0 + 16 + li;
+        const int c0 = col0 + li, c1 = col0 + 16 + li;
+        for (int k0 = 0; k0 < k; k0 += 4) {
+            const int kk = k0 + lk;
+            const bool kin = kk < k;
+            double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
+            if (kin) {
+                if (r0 < m) a0 = TA ? A[kk + (size_t)r0 * lda] : A[r0 + (size_t)kk * lda];
+                if (r1 < m) a1 = TA ? A[kk + (size_t)r1 * lda] : A[r1 + (size_t)kk * lda];
+                if (c0 < n) b0 = TB ? B[c0 + (size_t)kk * ldb] : B[kk + (size_t)c0 * ldb];
+                if (c1 < n) b1 = TB ? B[c1 + (size_t)kk * ldb] : B[kk + (size_t)c1 * ldb];
+            }
+            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+        }
+        const double alpha = g.alpha, beta = g.beta;
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi) {
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj) {
+                const int col = col0 + 16 * bj + li;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int row = row0 + 16 * bi + 4 * lk + i;
+                    if (row < m && col < n) {
+                        double *c = C + row + (size_t)col * ldc;
+                        // beta == 0: C is not read (BLAS semantics)
+                        *c = (beta == 0.0) ? alpha * acc[bi][bj][i] : fma(alpha, acc[bi][bj][i], beta * *c);
+                    }
+                }
+            }
+        }
+    }
+}
+
+// C = beta C when k == 0 or alpha == 0 (BLAS quick path)
+__global__ void gemm_scale_kernel(GemmSet g, const int *prefix, int fixed_tiles)
+{
+    const int total = prefix ? prefix[g.count] : fixed_tiles * g.count;
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+        int s, local;
+        if (prefix) {
+            int lo = 0, hi = g.count;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (prefix[mid] <= t) lo = mid; else hi = mid;
+            }
+            s = lo;
+            local = t - prefix[s];
+        } else {
+            s = t / fixed_tiles;
+            local = t - s * fixed_tiles;
+        }
+        const int id = g.index ? g.index[s] : s;
+        const int m = pick(g.m_arr, g.m, id), n = pick(g.n_arr, g.n, id);
+        const int ldc = pick(g.ldc_arr, g.ldc, id);
+        const int mt = (m + kTile - 1) / kTile;
+        const int row0 = (local % mt) * kTile, col0 = (local / mt) * kTile;
+        double *C = g.C[id];
+        for (int e = threadIdx.x; e < kTile * kTile; e += blockDim.x) {
+            const int row = row0 + e % kTile, col = col0 + e / kTile;
+            if (row < m && col < n) {
+                double *c = C + row + (size_t)col * ldc;
+                *c = (g.beta == 0.0) ? 0.0 : g.beta * *c;
+            }
+        }
+    }
+}
+
+}  // namespace
+
+// prefix: device tile prefix (vbatched) or null (fixed sizes).  max_tiles
+// bounds the flat tile count for the grid size.
+magma_int_t launch_gemm_tiles(const GemmSet &g, const int *prefix, long max_tiles, hipStream_t st)
+{
+    if (g.count <= 0 || max_tiles <= 0) return MAGMA_SUCCESS;
+    const int fixed_tiles = prefix ? 0 : ceil_div(g.m, kTile) * ceil_div(g.n, kTile);
+    // enough resident wavefronts to fill 256 CUs several times over
+    const long want = (max_tiles + kWaves - 1) / kWaves;
+    const int grid = (int)std::min<long>(want, 256L * 16);
+    const bool scale_only = (!prefix && (g.k == 0 || g.alpha == 0.0)) || g.alpha == 0.0;
+    if (scale_only) {
+        gemm_scale_kernel<<<grid, 256, 0, st>>>(g, prefix, fixed_tiles);
+    } else if (g.ta == 'N' && g.tb == 'N') {
+        gemm_mfma_kernel<false, false><<<grid, 64 * kWaves, 0, st>>>(g, prefix, fixed_tiles);
+    } else if (g.ta == 'N') {
+        gemm_mfma_kernel<false, true><<<grid, 64 * kWaves, 0, st>>>(g, prefix, fixed_tiles);
+    } else if (g.tb == 'N') {
+        gemm_mfma_kernel<true, false><<<grid, 64 * kWaves, 0, st>>>(g, prefix, fixed_tiles);
+    } else {
+        gemm_mfma_kernel<true, true><<<grid, 64 * kWaves, 0, st>>>(g, prefix, fixed_tiles);
+    }
+    MAGMA_LAUNCH_CHECK();
+    return MAGMA_SUCCESS;
+}
+
+}  // namespace magma_amd

@@ -1,0 +1,51 @@
+// kernels.h -- launch interfaces between the C-ABI drivers and the HIP kernels.
+#pragma once
+
+#include "internal.h"
+
+namespace magma_amd {
+
+// Describes a set of matrices for one launch.  Sizes either fixed (arrays
+// null) or per matrix (device arrays indexed by matrix id).  `index`, when not
+// null, lists the matrix ids this launch processes (a bucket of a vbatched
+// call); otherwise ids are 0..count-1.
+struct MatSet {
+    int m = 0, n = 0, lda = 0;                      // fixed sizes
+    const int *m_arr = nullptr, *n_arr = nullptr, *lda_arr = nullptr;
+    double **A = nullptr;
+    int **ipiv = nullptr;
+    int *info = nullptr;
+    // right-hand sides (solves)
+    int nrhs = 0, ldb = 0;
+    const int *nrhs_arr = nullptr, *ldb_arr = nullptr;
+    double **B = nullptr;
+    const int *index = nullptr;
+    int count = 0;
+};
+
+// LU with partial pivoting, LAPACK semantics (optionally followed by the
+// no-transpose solve of the right-hand sides in `B`).  Chooses the kernel
+// family from max_m / max_n.  `max_mn` bounds min(m,n) over the set.
+magma_int_t launch_getrf(const MatSet &s, int max_m, int max_n, bool fused_solve, hipStream_t st);
+// Cholesky, lower, LAPACK semantics (optionally fused with the solve).
+magma_int_t launch_potrf(const MatSet &s, int max_n, bool fused_solve, hipStream_t st);
+// Solves with existing factors.  trans: 'N' or 'T' (getrs); uplo 'L'/'U' (potrs).
+magma_int_t launch_getrs(const MatSet &s, char trans, int max_n, int max_nrhs, hipStream_t st);
+magma_int_t launch_potrs(const MatSet &s, char uplo, int max_n, int max_nrhs, hipStream_t st);
+
+// GEMM C = alpha op(A) op(B) + beta C for a set of (possibly different) sizes.
+struct GemmSet {
+    char ta = 'N', tb = 'N';
+    int m = 0, n = 0, k = 0, lda = 0, ldb = 0, ldc = 0;
+    const int *m_arr = nullptr, *n_arr = nullptr, *k_arr = nullptr;
+    const int *lda_arr = nullptr, *ldb_arr = nullptr, *ldc_arr = nullptr;
+    double alpha = 1.0, beta = 0.0;
+    const double *const *A = nullptr;
+    const double *const *B = nullptr;
+    double **C = nullptr;
+    const int *index = nullptr;
+    int count = 0;
+};
+magma_int_t launch_gemm(const GemmSet &g, int max_m, int max_n, int max_k, hipStream_t st);
+
+}  // namespace magma_amd

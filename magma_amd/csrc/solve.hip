@@ -1,0 +1,152 @@
+// solve.hip -- triangular solves with LU / Cholesky factors (getrs, potrs),
+// one 256-thread workgroup per matrix, right-hand sides staged in LDS in
+// chunks of up to kChunk columns.
+//
+// Replaces the reference's getrs/potrs chains (src/zgetrs_batched.cpp:86-273,
+// src/zpotrs_batched.cpp:70-249: laswp_rowserial + trsv, or trtri_diag +
+// trsm_outofplace GEMMs with a 128x128 dinvA workspace per matrix) with one
+// launch and no workspace.  Sweeps are column-oriented (axpy form, one barrier
+// per step): for the no-transpose L and U solves this is the reference BLAS
+// dtrsm loop order, so results are bit-identical to LAPACK's dgetrs /
+// dpotrs forward sweep; the transposed sweeps associate their sums in the
+// opposite order to dtrsm's dot-product form and agree to rounding.
+#include "kernels.h"
+
+namespace magma_amd {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kChunk = 8;
+
+__device__ __forceinline__ int pick(const int *arr, int fixed, int id) { return arr ? arr[id] : fixed; }
+
+// One triangular sweep over X (n x nc, LDS, leading dimension ldx) with factor T.
+//   lower/upper: which triangle of T is used; trans: op(T) = T';
+//   unit: implicit unit diagonal.  op(T) lower -> forward sweep, upper -> backward.
+__device__ void sweep(const double *T, int ldt, int n, double *X, int ldx, int nc,
+                      bool lower, bool trans, bool unit)
+{
+    const int tid = threadIdx.x;
+    const bool forward = (lower != trans);
+    for (int s = 0; s < n; ++s) {
+        const int k = forward ? s : n - 1 - s;
+        if (!unit) {
+            if (tid < nc) {
+                double *xk = X + (size_t)tid * ldx + k;
+                const double d = T[k + (size_t)k * ldt];
+                // no-transpose sweeps skip a zero entry like reference dtrsm;
+                // its transposed (dot-form) sweeps always divide
+                if (trans || *xk != 0.0) *xk = *xk / d;
+            }
+            __syncthreads();
+        }
+        // rows still to update: i > k (forward) or i < k (backward)
+        const int lo = forward ? k + 1 : 0;
+        const int hi = forward ? n : k;
+        for (int i = lo + tid; i < hi; i += kThreads) {
+            // op(T)(i,k): T(i,k) without transpose, T(k,i) with
+            const double t = trans ? T[k + (size_t)i * ldt] : T[i + (size_t)k * ldt];
+            for (int c = 0; c < nc; ++c) {
+                const double xk = X[(size_t)c * ldx + k];
+                X[(size_t)c * ldx + i] = fma(-xk, t, X[(size_t)c * ldx + i]);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// mode: 'N' getrs no-trans, 'T' getrs trans, 'L' potrs lower, 'U' potrs upper.
+__global__ void __launch_bounds__(kThreads) solve_kernel(MatSet s, char mode)
+{
+    extern __shared__ double X[];
+    const int item = blockIdx.x;
+    const int id = s.index ? s.index[item] : item;
+    const int n = pick(s.n_arr, s.n, id);
+    const int lda = pick(s.lda_arr, s.lda, id);
+    const int nrhs = pick(s.nrhs_arr, s.nrhs, id);
+    const int ldb = pick(s.ldb_arr, s.ldb, id);
+    const double *A = s.A[id];
+    double *B = s.B[id];
+    const int *ipiv = (mode == 'N' || mode == 'T') ? s.ipiv[id] : nullptr;
+    const int tid = threadIdx.x;
+    const int ldx = n;
+    if (n <= 0) return;
+    // after a fused factorization: no solve with a singular factor (dgesv/dposv)
+    if (s.info && s.info[id] != 0) return;
+
+    for (int c0 = 0; c0 < nrhs; c0 += kChunk) {
+        const int nc = min(kChunk, nrhs - c0);
+        for (int e = tid; e < n * nc; e += kThreads) {
+            const int i = e % n, c = e / n;
+            X[(size_t)c * ldx + i] = B[i + (size_t)(c0 + c) * ldb];
+        }
+        __syncthreads();
+        if (mode == 'N') {
+            // dlaswp forward: interchanges in order, one thread per column
+            if (tid < nc)
+                for (int k = 0; k < n; ++k) {
+                    const int p = ipiv[k] - 1;
+                    if (p != k && p >= 0 && p < n) {
+                        double *x = X + (size_t)tid * ldx;
+                        const double t = x[k];
+                        x[k] = x[p];
+                        x[p] = t;
+                    }
+                }
+            __syncthreads();
+            sweep(A, lda, n, X, ldx, nc, true, false, true);
+            sweep(A, lda, n, X, ldx, nc, false, false, false);
+        } else if (mode == 'T') {
+            sweep(A, lda, n, X, ldx, nc, false, true, false);
+            sweep(A, lda, n, X, ldx, nc, true, true, true);
+            if (tid < nc)
+                for (int k = n - 1; k >= 0; --k) {
+                    const int p = ipiv[k] - 1;
+                    if (p != k && p >= 0 && p < n) {
+                        double *x = X + (size_t)tid * ldx;
+                        const double t = x[k];
+                        x[k] = x[p];
+                        x[p] = t;
+                    }
+                }
+            __syncthreads();
+        } else if (mode == 'L') {
+            sweep(A, lda, n, X, ldx, nc, true, false, false);
+            sweep(A, lda, n, X, ldx, nc, true, true, false);
+        } else {
+            sweep(A, lda, n, X, ldx, nc, false, true, false);
+            sweep(A, lda, n, X, ldx, nc, false, false, false);
+        }
+        for (int e = tid; e < n * nc; e += kThreads) {
+            const int i = e % n, c = e / n;
+            B[i + (size_t)(c0 + c) * ldb] = X[(size_t)c * ldx + i];
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+static magma_int_t launch_solve(const MatSet &s, char mode, int max_n, hipStream_t st)
+{
+    if (s.count <= 0 || max_n <= 0) return MAGMA_SUCCESS;
+    const size_t lds = sizeof(double) * (size_t)max_n * kChunk;
+    if (lds > 64 * 1024) return MAGMA_ERR_NOT_SUPPORTED;
+    solve_kernel<<<s.count, kThreads, lds, st>>>(s, mode);
+    MAGMA_LAUNCH_CHECK();
+    return MAGMA_SUCCESS;
+}
+
+magma_int_t launch_getrs(const MatSet &s, char trans, int max_n, int max_nrhs, hipStream_t st)
+{
+    (void)max_nrhs;
+    return launch_solve(s, trans == 'N' ? 'N' : 'T', max_n, st);
+}
+
+magma_int_t launch_potrs(const MatSet &s, char uplo, int max_n, int max_nrhs, hipStream_t st)
+{
+    (void)max_nrhs;
+    return launch_solve(s, uplo == 'L' ? 'L' : 'U', max_n, st);
+}
+
+}  // namespace magma_amd

@@ -1,0 +1,123 @@
+"""Deterministic synthetic inputs for the batched factor-and-solve workloads.
+
+Every value is a pure function of (seed, matrix index, element index):
+
+    u = (splitmix64(seed * 2**40 + matrix * 2**20 + elem) >> 11) * 2**-53   in [0, 1)
+    uniform[-1, 1] = 2u - 1
+    N(0, 1) (Box-Muller) = sqrt(-2 ln(1 - u(2e))) * cos(2 pi u(2e + 1))
+
+Matrices are column-major: element (i, j) of an n x n matrix is elem i + j*n.
+Right-hand sides of a workload with matrix seed s use seed s + 100, the B and
+C operands of a GEMM seeds s + 100 and s + 200.  Per-matrix scalar draws (sizes,
+condition numbers) use elem = 2**20 - 1 of their matrix index.
+
+This mirrors the role of the reference testers' lapackf77_dlarnv / magma_zmake_hpd
+input generation (testing/testing_zgetrf_batched.cpp:93,133,
+testing/magma_zutil.cpp:49-58) with the distributions the benchmark
+configurations name.
+"""
+import numpy as np
+
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+SCALAR_ELEM = (1 << 20) - 1
+
+
+def splitmix64(x):
+    """Vectorised splitmix64 finaliser on uint64 arrays (wrapping arithmetic)."""
+    x = np.asarray(x, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = x + np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def u01(seed, mat, elem):
+    """u in [0,1) for broadcastable (seed, mat, elem)."""
+    seed = np.asarray(seed, dtype=np.uint64)
+    mat = np.asarray(mat, dtype=np.uint64)
+    elem = np.asarray(elem, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        key = (seed << np.uint64(40)) + (mat << np.uint64(20)) + elem
+    return (splitmix64(key) >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+
+
+def uniform_pm1(seed, mats, count):
+    """(len(mats), count) array of U[-1,1] values, elem = 0..count-1."""
+    mats = np.asarray(mats, dtype=np.uint64).reshape(-1, 1)
+    e = np.arange(count, dtype=np.uint64).reshape(1, -1)
+    return 2.0 * u01(seed, mats, e) - 1.0
+
+
+def normal(seed, mats, count):
+    """(len(mats), count) N(0,1) values by Box-Muller."""
+    mats = np.asarray(mats, dtype=np.uint64).reshape(-1, 1)
+    e = np.arange(count, dtype=np.uint64).reshape(1, -1)
+    u1 = u01(seed, mats, 2 * e)
+    u2 = u01(seed, mats, 2 * e + np.uint64(1))
+    return np.sqrt(-2.0 * np.log1p(-u1)) * np.cos(2.0 * np.pi * u2)
+
+
+def scalar_u(seed, mats):
+    """One u in [0,1) per matrix (sizes, condition numbers)."""
+    return u01(seed, np.asarray(mats, dtype=np.uint64), np.uint64(SCALAR_ELEM))
+
+
+def to_colmajor(A):
+    """(b, n, n) row-major numpy matrices -> (b, n, n) arrays whose C order is column-major."""
+    return np.ascontiguousarray(np.swapaxes(A, -1, -2))
+
+
+def general_batch(seed, batch, n, first=0):
+    """cfg1-style general matrices U[-1,1]; returns (batch, n, n) in column-major storage
+    (storage[b, j, i] = A_b(i, j))."""
+    mats = np.arange(first, first + batch)
+    return uniform_pm1(seed, mats, n * n).reshape(batch, n, n)
+
+
+def rhs_batch(seed, batch, n, nrhs, first=0):
+    """U[-1,1] right-hand sides, column-major (batch, nrhs, n) storage."""
+    mats = np.arange(first, first + batch)
+    return uniform_pm1(seed + 100, mats, n * nrhs).reshape(batch, nrhs, n)
+
+
+def spd_batch(seed, batch, n, first=0):
+    """cfg2-style SPD matrices A = B B' + n I with B ~ N(0,1); column-major storage
+    (symmetric, so storage order does not matter)."""
+    mats = np.arange(first, first + batch)
+    Bm = normal(seed, mats, n * n).reshape(batch, n, n)
+    A = Bm @ np.swapaxes(Bm, -1, -2)
+    A += n * np.eye(n)
+    return np.ascontiguousarray(A)
+
+
+def vbatch_sizes(seed, batch, lo, hi, first=0):
+    """Log-uniform integer sizes in [lo, hi] (cfg3 / cfg5 rule)."""
+    u = scalar_u(seed, np.arange(first, first + batch))
+    n = np.floor(np.exp(np.log(lo) + u * (np.log(hi + 1) - np.log(lo)))).astype(np.int64)
+    return np.clip(n, lo, hi)
+
+
+def cond_batch_params(seed, batch, n, first=0):
+    """cfg4: kappa = 10**(2 + 4u) and singular values s_i = kappa**(-i/(n-1))."""
+    u = scalar_u(seed, np.arange(first, first + batch))
+    kappa = 10.0 ** (2.0 + 4.0 * u)
+    i = np.arange(n, dtype=np.float64)
+    s = kappa.reshape(-1, 1) ** (-(i / (n - 1))).reshape(1, -1)
+    return kappa, s
+
+
+def cond_batch(seed, batch, n, first=0):
+    """cfg4 matrices A = U diag(s) V' with U, V the Q factors of QR of N(0,1)
+    matrices (elem 0..n^2-1 for U, n^2..2n^2-1 for V).  Returns column-major
+    (batch, n, n) storage and kappa.  Uses numpy's LAPACK QR (host)."""
+    mats = np.arange(first, first + batch)
+    G = normal(seed, mats, 2 * n * n)
+    G1 = G[:, : n * n].reshape(batch, n, n)       # storage order: column-major
+    G2 = G[:, n * n:].reshape(batch, n, n)
+    U, _ = np.linalg.qr(np.swapaxes(G1, -1, -2))
+    V, _ = np.linalg.qr(np.swapaxes(G2, -1, -2))
+    kappa, s = cond_batch_params(seed, batch, n, first)
+    A = (U * s[:, None, :]) @ np.swapaxes(V, -1, -2)
+    return to_colmajor(A), kappa

@@ -1,0 +1,228 @@
+"""ctypes wrapper of oracle/liboracle.so -- TEST INFRASTRUCTURE ONLY.
+
+Loaded by tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py
+as the checker; the product never imports it.
+
+Arrays follow the tests' storage convention: a batch of column-major matrices
+is a C-contiguous numpy array of shape (count, cols, ld) (storage[b, j, i] =
+A_b(i, j)); pivots are int32 (count, min(m, n)), 1-based.
+"""
+import ctypes
+import glob
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+_L = ctypes.c_int64
+_C = ctypes.c_char
+
+_PROTO = {
+    "orc_idamax": (_I, [_I, _P]),
+    "orc_dgetrf": (None, [_I, _I, _P, _I, _P, _P]),
+    "orc_dlaswp": (None, [_I, _P, _I, _I, _I, _P, _I]),
+    "orc_dgetrs": (None, [_C, _I, _I, _P, _I, _P, _P, _I]),
+    "orc_dpotrf": (None, [_C, _I, _P, _I, _P]),
+    "orc_dpotrs": (None, [_C, _I, _I, _P, _I, _P, _I]),
+    "orc_dgemm": (None, [_C, _C, _I, _I, _I, _D, _P, _I, _P, _I, _D, _P, _I]),
+    "orc_eps": (_D, []),
+    "orc_dlange": (_D, [_C, _I, _I, _P, _I]),
+    "orc_lu_error": (_D, [_I, _I, _P, _I, _P, _I, _P]),
+    "orc_solve_error": (_D, [_I, _I, _P, _I, _P, _I, _P, _I]),
+    "orc_chol_error": (_D, [_C, _I, _P, _I, _P, _I]),
+    "orc_batch_dgetrf": (None, [_I, _P, _I, _L, _P, _P, _I, _I]),
+    "orc_batch_dgesv": (None, [_I, _I, _P, _I, _L, _P, _P, _I, _L, _P, _I, _I]),
+    "orc_batch_dposv": (None, [_C, _I, _I, _P, _I, _L, _P, _I, _L, _P, _I, _I]),
+    "orc_vbatch_dgetrf": (None, [_P, _P, _P, _P, _P, _P, _P, _I, _I]),
+    "orc_vbatch_dgemm": (None, [_P, _P, _P, _D, _P, _P, _P, _P, _P, _P, _D, _P, _P, _P, _I, _I]),
+    "orc_lapack_open": (_I, [ctypes.c_char_p]),
+    "orc_lapack_config": (ctypes.c_char_p, []),
+    "orc_lapack_batch_dgesv": (_I, [_I, _I, _P, _I, _L, _P, _P, _I, _L, _P, _I, _I]),
+    "orc_lapack_batch_dposv": (_I, [_C, _I, _I, _P, _I, _L, _P, _I, _L, _P, _I, _I]),
+    "orc_lapack_batch_dgetrf": (_I, [_I, _P, _I, _L, _P, _P, _I, _I]),
+    "orc_wtime": (_D, []),
+    "orc_max_threads": (_I, []),
+}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing: run make -C oracle")
+        _lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _PROTO.items():
+            f = getattr(_lib, name)
+            f.restype = res
+            f.argtypes = args
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_P)
+
+
+def _f64(a):
+    a = np.require(a, dtype=np.float64, requirements=["C", "W"])
+    return a
+
+
+EPS = 2.0 ** -53
+
+
+def dgetrf(A, m=None, n=None):
+    """A: (cols, ld) column-major storage of one matrix (modified copy returned)."""
+    A = _f64(np.array(A, dtype=np.float64, copy=True))
+    cols, ld = A.shape
+    m = ld if m is None else m
+    n = cols if n is None else n
+    ipiv = np.zeros(max(min(m, n), 1), dtype=np.int32)
+    info = np.zeros(1, dtype=np.int32)
+    lib().orc_dgetrf(m, n, _p(A), ld, _p(ipiv), _p(info))
+    return A, ipiv[: min(m, n)], int(info[0])
+
+
+def dgetrs(LU, ipiv, B, trans=b"N"):
+    LU = _f64(np.array(LU, copy=True))
+    B = _f64(np.array(B, copy=True))
+    n = LU.shape[0]
+    nrhs, ldb = B.shape
+    ip = np.require(ipiv, dtype=np.int32, requirements=["C"])
+    lib().orc_dgetrs(trans, n, nrhs, _p(LU), LU.shape[1], _p(ip), _p(B), ldb)
+    return B
+
+
+def dpotrf(A, uplo=b"L"):
+    A = _f64(np.array(A, copy=True))
+    n, ld = A.shape
+    info = np.zeros(1, dtype=np.int32)
+    lib().orc_dpotrf(uplo, n, _p(A), ld, _p(info))
+    return A, int(info[0])
+
+
+def dpotrs(L, B, uplo=b"L"):
+    L = _f64(np.array(L, copy=True))
+    B = _f64(np.array(B, copy=True))
+    n, ld = L.shape
+    nrhs, ldb = B.shape
+    lib().orc_dpotrs(uplo, n, nrhs, _p(L), ld, _p(B), ldb)
+    return B
+
+
+def dgemm(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc):
+    A = _f64(np.array(A, copy=True)); B = _f64(np.array(B, copy=True)); C = _f64(np.array(C, copy=True))
+    lib().orc_dgemm(ta, tb, m, n, k, alpha, _p(A), lda, _p(B), ldb, beta, _p(C), ldc)
+    return C
+
+
+def idamax(x):
+    x = _f64(np.array(x, dtype=np.float64))
+    return lib().orc_idamax(len(x), _p(x))
+
+
+def lu_error(A, LU, ipiv, m=None, n=None):
+    A = _f64(np.array(A)); LU = _f64(np.array(LU))
+    ip = np.require(ipiv, dtype=np.int32, requirements=["C"])
+    cols, ld = A.shape
+    m = ld if m is None else m
+    n = cols if n is None else n
+    return lib().orc_lu_error(m, n, _p(A), ld, _p(LU), LU.shape[1], _p(ip))
+
+
+def solve_error(A, X, B):
+    A = _f64(np.array(A)); X = _f64(np.array(X)); B = _f64(np.array(B))
+    n, lda = A.shape
+    nrhs = X.shape[0]
+    return lib().orc_solve_error(n, nrhs, _p(A), lda, _p(X), X.shape[1], _p(B), B.shape[1])
+
+
+def chol_error(A, L, uplo=b"L"):
+    A = _f64(np.array(A)); L = _f64(np.array(L))
+    n, lda = A.shape
+    return lib().orc_chol_error(uplo, n, _p(A), lda, _p(L), L.shape[1])
+
+
+def batch_dgetrf(A, nthreads=None):
+    """A: (count, n, ld) column-major batch; returns (LU, ipiv, info)."""
+    A = _f64(np.array(A, copy=True))
+    count, n, ld = A.shape
+    ipiv = np.zeros((count, n), dtype=np.int32)
+    info = np.zeros(count, dtype=np.int32)
+    lib().orc_batch_dgetrf(n, _p(A), ld, n * ld, _p(ipiv), _p(info), count, nthreads or os.cpu_count())
+    return A, ipiv, info
+
+
+def batch_dgesv(A, B, nthreads=None):
+    A = _f64(np.array(A, copy=True)); B = _f64(np.array(B, copy=True))
+    count, n, ld = A.shape
+    _, nrhs, ldb = B.shape
+    ipiv = np.zeros((count, n), dtype=np.int32)
+    info = np.zeros(count, dtype=np.int32)
+    lib().orc_batch_dgesv(n, nrhs, _p(A), ld, n * ld, _p(ipiv), _p(B), ldb, nrhs * ldb,
+                          _p(info), count, nthreads or os.cpu_count())
+    return A, ipiv, B, info
+
+
+def batch_dposv(A, B, uplo=b"L", nthreads=None):
+    A = _f64(np.array(A, copy=True)); B = _f64(np.array(B, copy=True))
+    count, n, ld = A.shape
+    _, nrhs, ldb = B.shape
+    info = np.zeros(count, dtype=np.int32)
+    lib().orc_batch_dposv(uplo, n, nrhs, _p(A), ld, n * ld, _p(B), ldb, nrhs * ldb,
+                          _p(info), count, nthreads or os.cpu_count())
+    return A, B, info
+
+
+def find_openblas():
+    """Path of an LP64 OpenBLAS exporting scipy_dgetrf_ (scipy's bundle), or None."""
+    try:
+        import scipy
+        d = os.path.join(os.path.dirname(os.path.dirname(scipy.__file__)), "scipy.libs")
+        c = sorted(glob.glob(os.path.join(d, "libscipy_openblas-*.so")))
+        return c[0] if c else None
+    except Exception:
+        return None
+
+
+_lapack_ok = None
+
+
+def lapack():
+    """Opens the third-party LAPACK once; returns its config string or None."""
+    global _lapack_ok
+    if _lapack_ok is None:
+        path = find_openblas()
+        _lapack_ok = bool(path) and lib().orc_lapack_open(path.encode()) == 0
+    return lib().orc_lapack_config().decode() if _lapack_ok else None
+
+
+def lapack_batch_dgesv(A, B, nthreads=None):
+    if lapack() is None:
+        raise RuntimeError("no third-party LAPACK found")
+    A = _f64(np.array(A, copy=True)); B = _f64(np.array(B, copy=True))
+    count, n, ld = A.shape
+    _, nrhs, ldb = B.shape
+    ipiv = np.zeros((count, n), dtype=np.int32)
+    info = np.zeros(count, dtype=np.int32)
+    lib().orc_lapack_batch_dgesv(n, nrhs, _p(A), ld, n * ld, _p(ipiv), _p(B), ldb, nrhs * ldb,
+                                 _p(info), count, nthreads or os.cpu_count())
+    return A, ipiv, B, info
+
+
+def lapack_batch_dposv(A, B, uplo=b"L", nthreads=None):
+    if lapack() is None:
+        raise RuntimeError("no third-party LAPACK found")
+    A = _f64(np.array(A, copy=True)); B = _f64(np.array(B, copy=True))
+    count, n, ld = A.shape
+    _, nrhs, ldb = B.shape
+    info = np.zeros(count, dtype=np.int32)
+    lib().orc_lapack_batch_dposv(uplo, n, nrhs, _p(A), ld, n * ld, _p(B), ldb, nrhs * ldb,
+                                 _p(info), count, nthreads or os.cpu_count())
+    return A, B, info

@@ -1,0 +1,447 @@
+#!/usr/bin/env python3
+"""Benchmark of the batched small-matrix factor-and-solve path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+
+Prints ONE JSON line.  A step is one call sequence of the library's C API over
+one batch of synthetic inputs that are already resident in HBM (each step gets
+its own pre-staged copy, since factorizations work in place).
+
+Workloads (named by what they are):
+  cond256_mixed_vs_plain  8192 matrices n=256, cond log-uniform in [1e2, 1e6],
+                          nrhs=1: plain FP64 (magma_dgetrf_batched +
+                          magma_dgetrs_batched) and mixed precision
+                          (magma_dsgesv_batched); value = mixed solves/s
+  lu32_gesv               10000 general n=32, nrhs=1, magma_dgesv_batched
+  chol64_posv             20000 SPD n=64, nrhs=4, magma_dposv_batched
+  vbatch_lu_gemm          50000 log-uniform n in [8,256]: magma_dgetrf_vbatched
+                          + magmablas_dgemm_vbatched (C += A B)
+
+The roofline object covers the dominant kernel (the factorization), timed with
+HIP events on the queue's stream; the cpu_baseline object times the CPU oracle
+(the reference's CPU path restated: LAPACK-semantics routines in C looped over
+the batch under OpenMP) on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "solves/s at FP64 backward err ≤ n·eps (plain vs mixed-prec) + p50/p99 latency"
+HBM_PEAK = 8.0e12           # B/s
+FP64_PEAK = 78.6e12         # flop/s (vector and matrix FP64 on MI355X)
+EPS = 2.0 ** -53
+
+
+def getrf_flops(n):
+    return 2.0 * n ** 3 / 3.0 - n * n / 2.0 + 5.0 * n / 6.0
+
+
+def getrf_bytes(n):
+    return 16.0 * n * n + 4.0 * n + 4.0
+
+
+def getrs_bytes(n, r):
+    return 8.0 * n * n + 4.0 * n + 16.0 * n * r
+
+
+def potrf_flops(n):
+    return n ** 3 / 3.0 + n * n / 2.0 + n / 6.0
+
+
+def potrf_bytes(n):
+    return 8.0 * n * (n + 1) + 4.0
+
+
+def potrs_bytes(n, r):
+    return 4.0 * n * (n + 1) + 16.0 * n * r
+
+
+class Timer:
+    """HIP events recorded on the library queue's stream (torch's current stream)."""
+
+    def __init__(self, torch):
+        self.torch = torch
+        self.pairs = []
+
+    def start(self):
+        e = self.torch.cuda.Event(enable_timing=True)
+        e.record()
+        return e
+
+    def stop(self, e0):
+        e1 = self.torch.cuda.Event(enable_timing=True)
+        e1.record()
+        self.pairs.append((e0, e1))
+
+    def ms(self):
+        return [a.elapsed_time(b) for a, b in self.pairs]
+
+
+def cpu_threads():
+    try:
+        return len(os.sched_getaffinity(0))
+    except Exception:
+        return os.cpu_count() or 1
+
+
+# ---------------------------------------------------------------------------
+def run_fixed(args, torch, ma, L, q, make_inputs, step_fn, solves_per_step, kernel_units, unit_label):
+    """Generic driver: stage warmup+steps copies, time steps, return stats."""
+    copies = [make_inputs(i) for i in range(args.warmup + args.steps)]
+    torch.cuda.synchronize()
+    for i in range(args.warmup):
+        step_fn(copies[i], None)
+    torch.cuda.synchronize()
+    timer = Timer(torch)
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step_fn(copies[args.warmup + i], timer)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    step_ms = timer.ms()
+    return wall, step_ms, copies
+
+
+def workload_lu32(args, torch, ma, L, q):
+    from magma_amd import gen
+    count, n, nrhs = 10000, 32, 1
+    A0 = torch.from_numpy(gen.general_batch(0, count, n)).cuda()
+    B0 = torch.from_numpy(gen.rhs_batch(0, count, n, nrhs)).cuda()
+    ipiv = ma.IntBatch(count, n)
+    info = torch.zeros(count, dtype=torch.int32, device="cuda")
+
+    def make(i):
+        A = A0.clone(); B = B0.clone()
+        return (A, B, ma.ptr_array(A, count, n * n), ma.ptr_array(B, count, n * nrhs))
+
+    def step(c, timer):
+        A, B, pa, pb = c
+        e = timer.start() if timer else None
+        L.magma_dgesv_batched(n, nrhs, pa.data_ptr(), n, ipiv.ptr, pb.data_ptr(), n, info.data_ptr(), count,
+                              q.handle)
+        if timer:
+            timer.stop(e)
+
+    wall, step_ms, copies = run_fixed(args, torch, ma, L, q, make, step, count, count, "matrices")
+    # parity spot check of the last step against the oracle on a sample
+    check = {}
+    if args.check:
+        from oracle import pyoracle
+        A_h = A0[:200].cpu().numpy(); B_h = B0[:200].cpu().numpy()
+        LUo, ipo, Xo, info_o = pyoracle.batch_dgesv(A_h, B_h)
+        X = copies[-1][1][:200].cpu().numpy()
+        check = {"sample": 200, "ipiv_equal": bool(np.array_equal(ipiv.numpy()[:200], ipo)),
+                 "x_bitexact": bool(np.array_equal(X, Xo)),
+                 "max_backward_err_over_n_eps": float(max(pyoracle.solve_error(A_h[s], X[s], B_h[s])
+                                                          for s in range(0, 200, 20)))}
+    ms = wall / args.steps * 1e3
+    kern_ms = float(np.mean(step_ms))
+    byt = count * (getrf_bytes(n) + 16.0 * n * nrhs)     # fused factor + solve
+    return dict(value=count / (ms * 1e-3), unit="solves/s", ms=ms, dtype="f64",
+                config={"workload": "lu32_gesv", "batch": count, "n": n, "nrhs": nrhs,
+                        "api": "magma_dgesv_batched", "inputs": "U[-1,1] seed 0"},
+                roofline={"bound": "hbm", "achieved": byt / (kern_ms * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9,
+                          "unit": "GB/s", "kernel": "getrf_reg_kernel<32, fused solve>",
+                          "kernel_ms": kern_ms, "bytes_per_launch": byt},
+                check=check, step_ms=step_ms,
+                cpu=dict(kind="dgesv", n=n, nrhs=nrhs, gen=lambda k: (gen.general_batch(0, k, n),
+                                                                     gen.rhs_batch(0, k, n, nrhs))))
+
+
+def workload_chol64(args, torch, ma, L, q):
+    from magma_amd import gen
+    count, n, nrhs = 20000, 64, 4
+    A0 = torch.from_numpy(gen.spd_batch(1, count, n)).cuda()
+    B0 = torch.from_numpy(gen.rhs_batch(1, count, n, nrhs)).cuda()
+    info = torch.zeros(count, dtype=torch.int32, device="cuda")
+
+    def make(i):
+        A = A0.clone(); B = B0.clone()
+        return (A, B, ma.ptr_array(A, count, n * n), ma.ptr_array(B, count, n * nrhs))
+
+    def step(c, timer):
+        A, B, pa, pb = c
+        e = timer.start() if timer else None
+        L.magma_dposv_batched(ma.MagmaLower, n, nrhs, pa.data_ptr(), n, pb.data_ptr(), n, info.data_ptr(),
+                              count, q.handle)
+        if timer:
+            timer.stop(e)
+
+    wall, step_ms, copies = run_fixed(args, torch, ma, L, q, make, step, count, count, "matrices")
+    check = {}
+    if args.check:
+        from oracle import pyoracle
+        A_h = A0[:100].cpu().numpy(); B_h = B0[:100].cpu().numpy()
+        X = copies[-1][1][:100].cpu().numpy()
+        check = {"sample": 100, "info_all_zero": bool(int(info.abs().sum()) == 0),
+                 "max_backward_err_over_n_eps": float(max(pyoracle.solve_error(A_h[s], X[s], B_h[s])
+                                                          for s in range(0, 100, 10)))}
+    ms = wall / args.steps * 1e3
+    kern_ms = float(np.mean(step_ms))
+    byt = count * (potrf_bytes(n) + 16.0 * n * nrhs)
+    return dict(value=count / (ms * 1e-3), unit="solves/s", ms=ms, dtype="f64",
+                config={"workload": "chol64_posv", "batch": count, "n": n, "nrhs": nrhs,
+                        "api": "magma_dposv_batched", "inputs": "B B'+64 I, B~N(0,1) seed 1"},
+                roofline={"bound": "hbm", "achieved": byt / (kern_ms * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9,
+                          "unit": "GB/s", "kernel": "potrf_reg_kernel<64, fused solve>",
+                          "kernel_ms": kern_ms, "bytes_per_launch": byt},
+                check=check, step_ms=step_ms,
+                cpu=dict(kind="dposv", n=n, nrhs=nrhs, gen=lambda k: (gen.spd_batch(1, k, n),
+                                                                     gen.rhs_batch(1, k, n, nrhs))))
+
+
+def workload_cond256(args, torch, ma, L, q):
+    """cfg4: plain FP64 getrf+getrs vs mixed dsgesv on cond-controlled n=256."""
+    from magma_amd import gen
+    count, n, nrhs = args.batch or 8192, 256, 1
+    A0, kappa = gen.cond_batch_torch(3, count, n, "cuda")
+    B0 = gen.uniform_pm1_torch(103, 0, count, n * nrhs, "cuda").view(count, nrhs, n).contiguous()
+    ipiv = ma.IntBatch(count, n)
+    info = torch.zeros(count, dtype=torch.int32, device="cuda")
+    have_mixed = L.exported("magma_dsgesv_batched")
+    fact = Timer(torch)
+
+    def make(i):
+        A = A0.clone(); B = B0.clone()
+        return (A, B, ma.ptr_array(A, count, n * n), ma.ptr_array(B, count, n * nrhs))
+
+    def step_plain(c, timer):
+        A, B, pa, pb = c
+        e = fact.start() if timer else None
+        L.magma_dgetrf_batched(n, n, pa.data_ptr(), n, ipiv.ptr, info.data_ptr(), count, q.handle)
+        if timer:
+            fact.stop(e)
+        L.magma_dgetrs_batched(ma.MagmaNoTrans, n, nrhs, pa.data_ptr(), n, ipiv.ptr, pb.data_ptr(), n, count,
+                               q.handle)
+
+    res = {}
+    wall, _, copies = run_fixed(args, torch, ma, L, q, make, step_plain, count, count, "matrices")
+    plain_ms = wall / args.steps * 1e3
+    fact_ms = float(np.mean(fact.ms()))
+    # backward error of the plain solution on a sample
+    A_s = A0[:64].cpu().numpy(); B_s = B0[:64].cpu().numpy(); X_s = copies[-1][1][:64].cpu().numpy()
+    del copies
+    res["plain"] = dict(solves_per_s=count / (plain_ms * 1e-3), ms_per_step=plain_ms, factor_kernel_ms=fact_ms)
+    flops = count * getrf_flops(n)
+    roof = {"bound": "mfma", "achieved": flops / (fact_ms * 1e-3) / 1e12, "peak": FP64_PEAK / 1e12,
+            "unit": "TFLOP/s", "kernel": "getrf_blocked_kernel<1,16> (FP64 VALU fma; FP64 peak)",
+            "kernel_ms": fact_ms, "flops_per_launch": flops}
+    value, ms = res["plain"]["solves_per_s"], plain_ms
+    check = {}
+    if args.check:
+        from oracle import pyoracle
+        errs = [pyoracle.solve_error(A_s[s], X_s[s], B_s[s]) for s in range(0, 64, 8)]
+        check["plain_max_backward_err_over_n_eps"] = float(max(errs))
+    if have_mixed:
+        X = torch.zeros_like(B0)
+        px = ma.ptr_array(X, count, n * nrhs)
+        it = torch.zeros(count, dtype=torch.int32, device="cuda")
+        mix = Timer(torch)
+
+        def make_m(i):
+            # A is read-only unless a matrix falls back to FP64 (then it holds
+            # the LU factors), so every step gets its own copy as well
+            A = A0.clone()
+            return (A, B0, ma.ptr_array(A, count, n * n), ma.ptr_array(B0, count, n * nrhs))
+
+        def step_mixed(c, timer):
+            A, B, pa, pb = c
+            e = mix.start() if timer else None
+            L.magma_dsgesv_batched(n, nrhs, pa.data_ptr(), n, ipiv.ptr, pb.data_ptr(), n, px.data_ptr(), n,
+                                   info.data_ptr(), it.data_ptr(), count, q.handle)
+            if timer:
+                mix.stop(e)
+
+        wall_m, _, _ = run_fixed(args, torch, ma, L, q, make_m, step_mixed, count, count, "matrices")
+        mixed_ms = wall_m / args.steps * 1e3
+        iters = it.cpu().numpy()
+        res["mixed"] = dict(solves_per_s=count / (mixed_ms * 1e-3), ms_per_step=mixed_ms,
+                            kernel_ms=float(np.mean(mix.ms())),
+                            sweeps_mean=float(np.mean(np.maximum(iters, 0))),
+                            fallbacks=int(np.sum(iters < 0)))
+        res["mixed_over_plain"] = res["mixed"]["solves_per_s"] / res["plain"]["solves_per_s"]
+        value, ms = res["mixed"]["solves_per_s"], mixed_ms
+        if args.check:
+            from oracle import pyoracle
+            Xm = X[:64].cpu().numpy()
+            errs = [pyoracle.solve_error(A_s[s], Xm[s], B_s[s]) / n for s in range(64)]
+            check["mixed_max_backward_err_over_n_eps"] = float(max(errs) * n)
+            check["mixed_max_backward_err_ratio_to_n_eps"] = float(max(errs))
+        byt = count * (8.0 * n * n + 16.0 * n * nrhs + 8)
+        roof = {"bound": "hbm", "achieved": byt / (res["mixed"]["kernel_ms"] * 1e-3) / 1e9,
+                "peak": HBM_PEAK / 1e9, "unit": "GB/s", "kernel": "dsgesv (mixed factor + FP64 refinement)",
+                "kernel_ms": res["mixed"]["kernel_ms"], "bytes_per_launch": byt}
+    return dict(value=value, unit="solves/s", ms=ms, dtype="f64",
+                config={"workload": "cond256_mixed_vs_plain", "batch": count, "n": n, "nrhs": nrhs,
+                        "inputs": "A = U diag(s) V', U,V from QR of N(0,1) (seed 3), cond log-uniform [1e2,1e6]",
+                        "modes": res},
+                roofline=roof, check=check, step_ms=None,
+                cpu=dict(kind="dgesv", n=n, nrhs=nrhs, gen=lambda k: (A0[:k].cpu().numpy(),
+                                                                     B0[:k].cpu().numpy())))
+
+
+def workload_vbatch(args, torch, ma, L, q):
+    from magma_amd import gen
+    count = 50000
+    sizes = gen.vbatch_sizes(2, count, 8, 256).astype(np.int32)
+    lds = sizes.astype(np.int64)
+    offs = np.concatenate([[0], np.cumsum(lds * lds)[:-1]])
+    total = int((lds * lds).sum())
+    dev = "cuda"
+    Aflat0 = torch.empty(total, dtype=torch.float64, device=dev)
+    Bflat = torch.empty(total, dtype=torch.float64, device=dev)
+    Cflat = torch.empty(total, dtype=torch.float64, device=dev)
+    # fill per size class (same generator: seed 2 for A, 102 for B, 202 for C)
+    for nval in np.unique(sizes):
+        idx = np.nonzero(sizes == nval)[0]
+        nn = int(nval) * int(nval)
+        for seed, buf in ((2, Aflat0), (102, Bflat), (202, Cflat)):
+            chunk = max(1, (1 << 22) // nn)
+            for c0 in range(0, len(idx), chunk):
+                ids = idx[c0:c0 + chunk]
+                mats = torch.from_numpy(ids.astype(np.int64)).to(dev).view(-1, 1)
+                e = torch.arange(nn, dtype=torch.int64, device=dev).view(1, -1)
+                vals = 2.0 * gen.u01_torch(seed, mats, e) - 1.0
+                pos = torch.from_numpy(offs[ids]).to(dev).view(-1, 1) + e
+                buf[pos.view(-1)] = vals.view(-1)
+    d_n = ma.dev_ints(sizes)
+    ipiv_off = np.concatenate([[0], np.cumsum(lds)[:-1]])
+    ipiv = torch.zeros(int(lds.sum()), dtype=torch.int32, device=dev)
+    pip = torch.from_numpy(ipiv.data_ptr() + ipiv_off * 4).to(dev)
+    info = torch.zeros(count + 1, dtype=torch.int32, device=dev)
+    pb = torch.from_numpy(Bflat.data_ptr() + offs * 8).to(dev)
+    pc = torch.from_numpy(Cflat.data_ptr() + offs * 8).to(dev)
+    lu_t, gemm_t = Timer(torch), Timer(torch)
+
+    def make(i):
+        A = Aflat0.clone()
+        return (A, torch.from_numpy(A.data_ptr() + offs * 8).to(dev))
+
+    def step(c, timer):
+        A, pa = c
+        e = lu_t.start() if timer else None
+        L.magma_dgetrf_vbatched_max_nocheck(d_n.data_ptr(), d_n.data_ptr(), 256, pa.data_ptr(), d_n.data_ptr(),
+                                            pip.data_ptr(), info.data_ptr(), count, q.handle)
+        if timer:
+            lu_t.stop(e)
+            e = gemm_t.start()
+        L.magmablas_dgemm_vbatched_max_nocheck(ma.MagmaNoTrans, ma.MagmaNoTrans, d_n.data_ptr(), d_n.data_ptr(),
+                                               d_n.data_ptr(), 1.0, pa.data_ptr(), d_n.data_ptr(), pb.data_ptr(),
+                                               d_n.data_ptr(), 1.0, pc.data_ptr(), d_n.data_ptr(), count, 256,
+                                               256, 256, q.handle)
+        if timer:
+            gemm_t.stop(e)
+
+    wall, _, _ = run_fixed(args, torch, ma, L, q, make, step, count, count, "matrices")
+    ms = wall / args.steps * 1e3
+    lu_ms = float(np.mean(lu_t.ms())); gemm_ms = float(np.mean(gemm_t.ms()))
+    flops_lu = float(sum(getrf_flops(int(x)) for x in sizes))
+    flops_gemm = float(np.sum(2.0 * lds ** 3))
+    return dict(value=count / (ms * 1e-3), unit="factorizations+gemms/s", ms=ms, dtype="f64",
+                config={"workload": "vbatch_lu_gemm", "batch": count, "n": "log-uniform [8,256] seed 2",
+                        "lu_ms": lu_ms, "gemm_ms": gemm_ms,
+                        "lu_tflops": flops_lu / (lu_ms * 1e-3) / 1e12,
+                        "gemm_tflops": flops_gemm / (gemm_ms * 1e-3) / 1e12},
+                roofline={"bound": "mfma", "achieved": flops_lu / (lu_ms * 1e-3) / 1e12,
+                          "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s", "kernel": "getrf_vbatched (all classes)",
+                          "kernel_ms": lu_ms, "flops_per_launch": flops_lu},
+                check={}, step_ms=None, cpu=None)
+
+
+WORKLOADS = {
+    "cond256_mixed_vs_plain": workload_cond256,
+    "lu32_gesv": workload_lu32,
+    "chol64_posv": workload_chol64,
+    "vbatch_lu_gemm": workload_vbatch,
+}
+
+
+def cpu_baseline(spec, seconds):
+    """Times the oracle (C, OpenMP over the batch) and, if present, a
+    third-party LAPACK on a bounded sample of the same workload."""
+    if spec is None:
+        return None
+    from oracle import pyoracle
+    threads = cpu_threads()
+    kind, n, nrhs = spec["kind"], spec["n"], spec["nrhs"]
+    fn = pyoracle.batch_dgesv if kind == "dgesv" else pyoracle.batch_dposv
+    lfn = pyoracle.lapack_batch_dgesv if kind == "dgesv" else pyoracle.lapack_batch_dposv
+    k = max(threads, 8)
+    A, B = spec["gen"](k)
+    t0 = time.perf_counter(); fn(A, B, nthreads=threads); dt = time.perf_counter() - t0
+    k2 = int(min(max(k, k * seconds / max(dt, 1e-6)), 20000))
+    A, B = spec["gen"](k2)
+    t0 = time.perf_counter(); fn(A, B, nthreads=threads); dt = time.perf_counter() - t0
+    out = {"value": k2 / dt, "unit": "solves/s", "cores": threads, "kind": "port",
+           "sample": f"{k2} matrices of the workload (n={n}, nrhs={nrhs}), oracle {kind} "
+                     f"(netlib-order C restatement) under OpenMP, {threads} threads, {dt:.1f} s"}
+    cfg = pyoracle.lapack()
+    if cfg:
+        t0 = time.perf_counter(); lfn(A, B, nthreads=threads); dl = time.perf_counter() - t0
+        out["lapack"] = {"value": k2 / dl, "unit": "solves/s", "library": cfg.split()[0] if cfg else "",
+                         "config": cfg, "cores": threads}
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="cond256_mixed_vs_plain", choices=sorted(WORKLOADS))
+    ap.add_argument("--batch", type=int, default=0, help="override the batch count (cond256 only)")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-check", dest="check", action="store_false")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on exactly one GPU: the batched path has no cross-GPU step "
+              "(every matrix is independent); run one process per GPU and add the rates.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import magma_amd as ma
+    L = ma.load()
+    q = ma.Queue(0)
+    L.magma_queue_set_async(q.handle, 1)      # back-to-back submission; synchronize around the timed region
+
+    r = WORKLOADS[args.workload](args, torch, ma, L, q)
+    roof = r["roofline"]
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    roof.setdefault("traffic", None)
+    cpu = None if args.no_cpu_baseline else cpu_baseline(r.get("cpu"), args.cpu_seconds)
+    lat = {}
+    if r.get("step_ms"):
+        sm = np.asarray(r["step_ms"])
+        lat = {"step_p50_ms": float(np.percentile(sm, 50)), "step_p99_ms": float(np.percentile(sm, 99))}
+    out = {
+        "metric": METRIC,
+        "value": r["value"],
+        "unit": r["unit"],
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": r["dtype"],
+        "data": "synthetic (deterministic splitmix64 generator, inputs resident in HBM)",
+        "config": dict(r["config"], parallelism="single GPU, batch-parallel", **lat),
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "check": r["check"],
+    }
+    print(json.dumps(out), flush=True)
+    q.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -60,6 +60,12 @@ void magmablas_dgemm_vbatched_nocheck(magma_trans_t transA, magma_trans_t transB
 magma_int_t magma_dgetrf_vbatched(magma_int_t *m, magma_int_t *n, double **dA_array, magma_int_t *ldda,
                                   magma_int_t **dipiv_array, magma_int_t *info_array,
                                   magma_int_t batchCount, magma_queue_t queue);
+/* New: the same without argument checks and without any host synchronization
+ * (graph-capturable); max_mn bounds max(m[i], n[i]). */
+magma_int_t magma_dgetrf_vbatched_max_nocheck(magma_int_t *m, magma_int_t *n, magma_int_t max_mn,
+                                              double **dA_array, magma_int_t *ldda,
+                                              magma_int_t **dipiv_array, magma_int_t *info_array,
+                                              magma_int_t batchCount, magma_queue_t queue);
 /* New: solves with the getrf_vbatched / potrf_vbatched factors. */
 magma_int_t magma_dgetrs_vbatched(magma_trans_t trans, magma_int_t *n, magma_int_t *nrhs,
                                   double **dA_array, magma_int_t *ldda, magma_int_t **dipiv_array,

@@ -68,6 +68,11 @@ void magma_queue_create_from_hip_internal(magma_device_t device, magma_hip_strea
 #define magma_queue_create_from_hip(device, stream, queue_ptr) \
         magma_queue_create_from_hip_internal(device, stream, queue_ptr, __func__, __FILE__, __LINE__)
 magma_hip_stream_t magma_queue_get_hip_stream(magma_queue_t queue);
+/* Extension: with async != 0 the factor/solve routines return as soon as their
+ * work is enqueued instead of synchronizing the queue (the reference's
+ * behaviour, kept as the default).  Needed for HIP graph capture and for
+ * back-to-back submission; the caller then synchronizes. */
+void magma_queue_set_async(magma_queue_t queue, magma_int_t async);
 
 /* ---- events (include/magma_auxiliary.h:284-300) */
 void magma_event_create(magma_event_t *event_ptr);

@@ -114,6 +114,75 @@ class IntBatch:
         return self.ptrs.data_ptr()
 
 
+class VarBatch:
+    """Matrices of different sizes (rows_i x cols_i, leading dimension ld_i =
+    max(rows_i, 1)) packed back to back in one device buffer, with the device
+    pointer array and device size arrays (length count + 1, the vbatched
+    convention) the C API takes."""
+
+    def __init__(self, mats, device="cuda"):
+        torch = _torch()
+        self.count = len(mats)
+        self.rows = np.array([m.shape[1] if m.ndim == 2 else 0 for m in mats], dtype=np.int32)
+        self.cols = np.array([m.shape[0] if m.ndim == 2 else 0 for m in mats], dtype=np.int32)
+        self.ld = np.maximum(self.rows, 1).astype(np.int32)
+        sizes = (self.ld.astype(np.int64) * np.maximum(self.cols, 1))
+        self.offsets = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+        flat = np.zeros(int(sizes.sum()) if self.count else 1, dtype=np.float64)
+        for i, m in enumerate(mats):
+            r, c = self.rows[i], self.cols[i]
+            if r and c:
+                blk = np.zeros((c, self.ld[i]))
+                blk[:, :r] = m
+                flat[self.offsets[i]: self.offsets[i] + blk.size] = blk.ravel()
+        self.t = torch.from_numpy(flat).to(device)
+        base = self.t.data_ptr()
+        self.ptrs = torch.from_numpy(base + self.offsets * 8).to(device)
+        self.d_rows = dev_ints(self.rows, device)
+        self.d_cols = dev_ints(self.cols, device)
+        self.d_ld = dev_ints(self.ld, device)
+
+    def matrices(self):
+        flat = self.t.cpu().numpy()
+        out = []
+        for i in range(self.count):
+            r, c, ld, o = self.rows[i], self.cols[i], self.ld[i], self.offsets[i]
+            out.append(flat[o: o + ld * c].reshape(c, ld)[:, :r].copy() if r and c else np.zeros((c, r)))
+        return out
+
+    @property
+    def ptr(self):
+        return self.ptrs.data_ptr()
+
+
+def dev_ints(values, device="cuda"):
+    """int32 device array of len(values) + 1 (the extra slot is library scratch)."""
+    torch = _torch()
+    a = np.zeros(len(values) + 1, dtype=np.int32)
+    a[: len(values)] = values
+    return torch.from_numpy(a).to(device)
+
+
+class VarInts:
+    """Per-matrix int32 vectors of lengths `lengths`, with a pointer array."""
+
+    def __init__(self, lengths, device="cuda"):
+        torch = _torch()
+        lengths = np.maximum(np.asarray(lengths, dtype=np.int64), 1)
+        self.offsets = np.concatenate([[0], np.cumsum(lengths)[:-1]])
+        self.lengths = lengths
+        self.t = torch.zeros(int(lengths.sum()), dtype=torch.int32, device=device)
+        self.ptrs = torch.from_numpy(self.t.data_ptr() + self.offsets * 4).to(device)
+
+    def numpy(self, i, n):
+        flat = self.t.cpu().numpy()
+        return flat[self.offsets[i]: self.offsets[i] + n]
+
+    @property
+    def ptr(self):
+        return self.ptrs.data_ptr()
+
+
 def _vec(count, device="cuda"):
     return _torch().zeros(count, dtype=_torch().int32, device=device)
 

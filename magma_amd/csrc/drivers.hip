@@ -57,7 +57,7 @@ static char trans_char(magma_trans_t t) { return t == MagmaNoTrans ? 'N' : 'T'; 
 
 static magma_int_t finish(magma_int_t r, magma_queue_t queue)
 {
-    if (r == MAGMA_SUCCESS) {
+    if (r == MAGMA_SUCCESS && !(queue && queue->async)) {
         hipError_t e = hipStreamSynchronize(stream_of(queue));
         if (e != hipSuccess) return report_hip_error(e, "hipStreamSynchronize", __FILE__, __LINE__);
     }
@@ -75,17 +75,23 @@ magma_int_t magma_dgetrf_batched(magma_int_t m, magma_int_t n, double **dA_array
                                  magma_int_t **ipiv_array, magma_int_t *info_array,
                                  magma_int_t batchCount, magma_queue_t queue)
 {
+    // info_array is zeroed on entry as the reference does
+    // (src/zgetrf_batched.cpp:94); on the factorization path the kernels
+    // write every entry, so only the early returns clear it.
     hipStream_t st = stream_of(queue);
-    zero_info(info_array, batchCount, st);
     magma_int_t arginfo = 0;
     if (m < 0) arginfo = -1;
     else if (n < 0) arginfo = -2;
     else if (ldda < std::max(1, m)) arginfo = -4;
     if (arginfo != 0) {
+        zero_info(info_array, batchCount, st);
         magma_xerbla(__func__, -arginfo);
         return arginfo;
     }
-    if (m == 0 || n == 0 || batchCount <= 0) return finish(MAGMA_SUCCESS, queue);
+    if (m == 0 || n == 0 || batchCount <= 0) {
+        zero_info(info_array, batchCount, st);
+        return finish(MAGMA_SUCCESS, queue);
+    }
     MatSet s;
     s.m = m; s.n = n; s.lda = ldda;
     s.A = dA_array; s.ipiv = ipiv_array; s.info = info_array;
@@ -150,7 +156,6 @@ magma_int_t magma_dpotrf_batched(magma_uplo_t uplo, magma_int_t n, double **dA_a
                                  magma_int_t *info_array, magma_int_t batchCount, magma_queue_t queue)
 {
     hipStream_t st = stream_of(queue);
-    zero_info(info_array, batchCount, st);
     magma_int_t arginfo = 0;
     if (uplo != MagmaUpper && uplo != MagmaLower) arginfo = -1;
     else if (n < 0) arginfo = -2;
@@ -160,10 +165,14 @@ magma_int_t magma_dpotrf_batched(magma_uplo_t uplo, magma_int_t n, double **dA_a
         arginfo = -1;
     }
     if (arginfo != 0) {
+        zero_info(info_array, batchCount, st);
         magma_xerbla(__func__, -arginfo);
         return arginfo;
     }
-    if (n == 0 || batchCount <= 0) return finish(MAGMA_SUCCESS, queue);
+    if (n == 0 || batchCount <= 0) {
+        zero_info(info_array, batchCount, st);
+        return finish(MAGMA_SUCCESS, queue);
+    }
     MatSet s;
     s.n = n; s.m = n; s.lda = lda;
     s.A = dA_array; s.info = info_array;

@@ -10,9 +10,10 @@
 // (magmablas/gemm_template_kernel_vbatched.cuh:18-304,
 // magmablas/dgemm_vbatched_core.cu:27-266).
 //
-// MFMA operand layout (16x16x4, wave64): lane l supplies A(l%16, l/16) and
-// B(l/16, l%16) of the 16x4 / 4x16 blocks and receives D(4*(l/16)+i, l%16),
-// i = 0..3, of the 16x16 result.
+// MFMA operand layout (v_mfma_f64_16x16x4f64, wave64): lane l supplies
+// A(l%16, l/16) and B(l/16, l%16) of the 16x4 / 4x16 blocks and receives
+// D(l/16 + 4*i, l%16), i = 0..3, of the 16x16 result (FP64 interleaves the
+// result rows, unlike the FP16/FP32 variants' 4*(l/16) + i).
 #include "kernels.h"
 #include "workqueue.h"
 
@@ -92,7 +93,7 @@ __global__ void __launch_bounds__(64 * kWaves) gemm_mfma_kernel(GemmSet g, const
                 const int col = col0 + 16 * bj + li;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    const int row = row0 + 16 * bi + 4 * lk + i;
+                    const int row = row0 + 16 * bi + lk + 4 * i;
                     if (row < m && col < n) {
                         double *c = C + row + (size_t)col * ldc;
                         // beta == 0: C is not read (BLAS semantics)

@@ -21,6 +21,7 @@ struct magma_queue {
     size_t work_bytes = 0;
     void *host = nullptr;          // pinned host staging
     size_t host_bytes = 0;
+    bool async = false;            // skip the end-of-routine synchronization
 };
 
 namespace magma_amd {

@@ -20,8 +20,25 @@ struct MatSet {
     const int *nrhs_arr = nullptr, *ldb_arr = nullptr;
     double **B = nullptr;
     const int *index = nullptr;
+    // Device-resident work list (vbatched buckets): when dev_count is set the
+    // number of items is *dev_count (count is then only an upper bound used
+    // to size the grid) and the id list starts at index + *dev_offset.
+    const int *dev_count = nullptr;
+    const int *dev_offset = nullptr;
     int count = 0;
 };
+
+#ifdef __HIPCC__
+__device__ __forceinline__ int item_count(const MatSet &s) { return s.dev_count ? *s.dev_count : s.count; }
+__device__ __forceinline__ const int *item_index(const MatSet &s)
+{
+    return s.index ? s.index + (s.dev_offset ? *s.dev_offset : 0) : nullptr;
+}
+#endif
+
+// Grid size for a persistent launch of `items` work units of which at most
+// `per_cu` are resident per compute unit.
+int persistent_grid(long items, int per_cu);
 
 // LU with partial pivoting, LAPACK semantics (optionally followed by the
 // no-transpose solve of the right-hand sides in `B`).  Chooses the kernel

@@ -114,7 +114,7 @@ __device__ void trailing_update(double *A, int lda, int r0, int m, int c0, int n
 
 // ---------------------------------------------------------------------------
 template <int RPT, int PW>
-__global__ void __launch_bounds__(kThreads) getrf_blocked_kernel(MatSet s)
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) getrf_blocked_kernel(MatSet s)
 {
     static_assert(PW <= 16, "panel width");
     __shared__ double prow[PW];
@@ -128,142 +128,161 @@ __global__ void __launch_bounds__(kThreads) getrf_blocked_kernel(MatSet s)
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int w = tid >> 6;
-    const int item = blockIdx.x;
-    const int id = s.index ? s.index[item] : item;
-    const int m = pick(s.m_arr, s.m, id);
-    const int n = pick(s.n_arr, s.n, id);
-    const int lda = pick(s.lda_arr, s.lda, id);
-    double *A = s.A[id];
-    int *ipiv = s.ipiv[id];
-    const int mn = min(m, n);
-    int info = 0;
+    const int cnt = item_count(s);
+    const int *idx = item_index(s);
+    for (int item = blockIdx.x; item < cnt; item += gridDim.x) {
+        const int id = idx ? idx[item] : item;
+        const int m = pick(s.m_arr, s.m, id);
+        const int n = pick(s.n_arr, s.n, id);
+        const int lda = pick(s.lda_arr, s.lda, id);
+        double *A = s.A[id];
+        int *ipiv = s.ipiv[id];
+        const int mn = min(m, n);
+        int info = 0;
 
-    for (int j0 = 0; j0 < mn; j0 += PW) {
-        const int jb = min(PW, mn - j0);
-        const int mp = m - j0;
-        double a[RPT][PW];
-        int pos[RPT];
-#pragma unroll
-        for (int i = 0; i < RPT; ++i) {
-            const int pr = tid + kThreads * i;
-            pos[i] = pr;
-#pragma unroll
-            for (int c = 0; c < PW; ++c)
-                a[i][c] = (pr < mp && c < jb) ? A[(j0 + pr) + (size_t)(j0 + c) * lda] : 0.0;
-        }
-        bool bad = false;
-#pragma unroll
-        for (int jj = 0; jj < PW; ++jj) {
-            if (jj < jb) {
-                double key = -1.0;
-                int kp = INT_MAX;
-#pragma unroll
-                for (int i = 0; i < RPT; ++i) {
-                    const int pr = tid + kThreads * i;
-                    const bool cand = pr < mp && pos[i] >= jj;
-                    const double v = a[i][jj];
-                    double k2 = -1.0;
-                    if (cand) k2 = (v != v) ? ((pos[i] == jj) ? __builtin_huge_val() : -1.0) : fabs(v);
-                    pivot_combine(key, kp, k2, cand ? pos[i] : INT_MAX);
-                }
-#pragma unroll
-                for (int off = 32; off > 0; off >>= 1)
-                    pivot_combine(key, kp, __shfl_xor(key, off), __shfl_xor(kp, off));
-                if (lane == 0) {
-                    red_key[w] = key;
-                    red_pos[w] = kp;
-                }
-                __syncthreads();
-                key = red_key[0];
-                kp = red_pos[0];
-#pragma unroll
-                for (int q = 1; q < kThreads / 64; ++q) pivot_combine(key, kp, red_key[q], red_pos[q]);
-                const int p = kp;
-#pragma unroll
-                for (int i = 0; i < RPT; ++i) {
-                    const int pr = tid + kThreads * i;
-                    if (pr < mp && pos[i] == p) {
-#pragma unroll
-                        for (int c = jj; c < PW; ++c) prow[c] = a[i][c];
+        for (int j0 = 0; j0 < mn; j0 += PW) {
+            const int jb = min(PW, mn - j0);
+            const int mp = m - j0;
+            for (int e = tid; e < PW * (PW + 1); e += kThreads) (&L11[0][0])[e] = 0.0;
+            double a[RPT][PW];
+            int pos[RPT];
+    #pragma unroll
+            for (int i = 0; i < RPT; ++i) {
+                const int pr = tid + kThreads * i;
+                pos[i] = pr;
+    #pragma unroll
+                for (int c = 0; c < PW; ++c)
+                    a[i][c] = (pr < mp && c < jb) ? A[(j0 + pr) + (size_t)(j0 + c) * lda] : 0.0;
+            }
+            bool bad = false;
+    #pragma unroll
+            for (int jj = 0; jj < PW; ++jj) {
+                if (jj < jb) {
+                    double key = -1.0;
+                    int kp = INT_MAX;
+    #pragma unroll
+                    for (int i = 0; i < RPT; ++i) {
+                        const int pr = tid + kThreads * i;
+                        const bool cand = pr < mp && pos[i] >= jj;
+                        const double v = a[i][jj];
+                        double k2 = -1.0;
+                        if (cand) k2 = (v != v) ? ((pos[i] == jj) ? __builtin_huge_val() : -1.0) : fabs(v);
+                        pivot_combine(key, kp, k2, cand ? pos[i] : INT_MAX);
                     }
-                }
-                if (tid == 0) swaps[jj] = p;
-                __syncthreads();
-                const double piv = prow[jj];
-                if (piv == 0.0 && info == 0) info = j0 + jj + 1;
-                const bool use_rcp = fabs(piv) >= DBL_MIN;
-                const double rc = 1.0 / piv;
-#pragma unroll
-                for (int i = 0; i < RPT; ++i) {
-                    const int pr = tid + kThreads * i;
-                    pos[i] = (pos[i] == jj) ? p : ((pos[i] == p) ? jj : pos[i]);
-                    const bool below = pr < mp && pos[i] > jj;
-                    if (below) {
-                        double l = a[i][jj];
-                        if (piv != 0.0) l = use_rcp ? rc * l : l / piv;
-                        a[i][jj] = l;
-                        bad |= !__builtin_isfinite(l);
-#pragma unroll
-                        for (int c = jj + 1; c < PW; ++c) {
-                            const double u = prow[c];
-                            if (c < jb && u != 0.0) a[i][c] = fma(l, -u, a[i][c]);
+    #pragma unroll
+                    for (int off = 32; off > 0; off >>= 1)
+                        pivot_combine(key, kp, __shfl_xor(key, off), __shfl_xor(kp, off));
+                    if (lane == 0) {
+                        red_key[w] = key;
+                        red_pos[w] = kp;
+                    }
+                    __syncthreads();
+                    key = red_key[0];
+                    kp = red_pos[0];
+    #pragma unroll
+                    for (int q = 1; q < kThreads / 64; ++q) pivot_combine(key, kp, red_key[q], red_pos[q]);
+                    const int p = kp;
+    #pragma unroll
+                    for (int i = 0; i < RPT; ++i) {
+                        const int pr = tid + kThreads * i;
+                        if (pr < mp && pos[i] == p) {
+    #pragma unroll
+                            for (int c = jj; c < PW; ++c) prow[c] = a[i][c];
+                        }
+                    }
+                    if (tid == 0) swaps[jj] = p;
+                    __syncthreads();
+                    const double piv = prow[jj];
+                    if (piv == 0.0 && info == 0) info = j0 + jj + 1;
+                    const bool use_rcp = fabs(piv) >= DBL_MIN;
+                    const double rc = 1.0 / piv;
+    #pragma unroll
+                    for (int i = 0; i < RPT; ++i) {
+                        const int pr = tid + kThreads * i;
+                        pos[i] = (pos[i] == jj) ? p : ((pos[i] == p) ? jj : pos[i]);
+                        const bool below = pr < mp && pos[i] > jj;
+                        if (below) {
+                            double l = a[i][jj];
+                            if (piv != 0.0) l = use_rcp ? rc * l : l / piv;
+                            a[i][jj] = l;
+                            bad |= !__builtin_isfinite(l);
+    #pragma unroll
+                            for (int c = jj + 1; c < PW; ++c) {
+                                const double u = prow[c];
+                                if (c < jb && u != 0.0) a[i][c] = fma(l, -u, a[i][c]);
+                            }
                         }
                     }
                 }
             }
-        }
-        // panel back to global in pivoted order; L11 (unit lower) to LDS
-#pragma unroll
-        for (int i = 0; i < RPT; ++i) {
-            const int pr = tid + kThreads * i;
-            if (pr < mp) {
-#pragma unroll
-                for (int c = 0; c < PW; ++c)
-                    if (c < jb) A[(j0 + pos[i]) + (size_t)(j0 + c) * lda] = a[i][c];
-                if (pos[i] < jb) {
-#pragma unroll
-                    for (int c = 0; c < PW; ++c) L11[pos[i]][c] = a[i][c];
+            // panel back to global in pivoted order; L11 (unit lower) to LDS
+    #pragma unroll
+            for (int i = 0; i < RPT; ++i) {
+                const int pr = tid + kThreads * i;
+                if (pr < mp) {
+    #pragma unroll
+                    for (int c = 0; c < PW; ++c)
+                        if (c < jb) A[(j0 + pos[i]) + (size_t)(j0 + c) * lda] = a[i][c];
+                    if (pos[i] < jb) {
+    #pragma unroll
+                        for (int c = 0; c < PW; ++c) L11[pos[i]][c] = a[i][c];
+                    }
                 }
             }
-        }
-        if (tid < jb) ipiv[j0 + tid] = j0 + swaps[tid] + 1;
-        const bool exact = __syncthreads_or(bad);
+            if (tid < jb) ipiv[j0 + tid] = j0 + swaps[tid] + 1;
+            const bool exact = __syncthreads_or(bad);
 
-        // laswp on the other columns, then U12 = L11^{-1} A12 on the right ones
-        const int nother = n - jb;
-        for (int t = tid; t < nother; t += kThreads) {
-            const int c = (t < j0) ? t : t + jb;
-            double *col = A + (size_t)c * lda;
-            for (int jj = 0; jj < jb; ++jj) {
-                const int r2 = j0 + swaps[jj];
-                const int r1 = j0 + jj;
-                if (r2 != r1) {
-                    const double x = col[r1];
-                    col[r1] = col[r2];
-                    col[r2] = x;
+            // laswp on the other columns, then U12 = L11^{-1} A12 on the right ones
+            const int nother = n - jb;
+            for (int t = tid; t < nother; t += kThreads) {
+                const int c = (t < j0) ? t : t + jb;
+                double *col = A + (size_t)c * lda;
+                for (int jj = 0; jj < jb; ++jj) {
+                    const int r2 = j0 + swaps[jj];
+                    const int r1 = j0 + jj;
+                    if (r2 != r1) {
+                        const double x = col[r1];
+                        col[r1] = col[r2];
+                        col[r2] = x;
+                    }
+                }
+                if (c >= j0 + jb) {
+                    double x[PW];
+    #pragma unroll
+                    for (int k = 0; k < PW; ++k) x[k] = (k < jb) ? col[j0 + k] : 0.0;
+                    // rows of L11 at and beyond jb are zero, so the last
+                    // (narrower) panel needs no guard; one L11 column per
+                    // step keeps the register footprint at the column itself
+                    if (!exact) {
+    #pragma unroll
+                        for (int k = 0; k < PW; ++k) {
+                            const double u = -x[k];
+    #pragma unroll
+                            for (int i = k + 1; i < PW; ++i) x[i] = fma(L11[i][k], u, x[i]);
+                            __builtin_amdgcn_sched_barrier(0);
+                        }
+                    } else {
+    #pragma unroll
+                        for (int k = 0; k < PW; ++k) {
+                            const double u = x[k];
+    #pragma unroll
+                            for (int i = k + 1; i < PW; ++i)
+                                if (u != 0.0) x[i] = fma(L11[i][k], -u, x[i]);
+                            __builtin_amdgcn_sched_barrier(0);
+                        }
+                    }
+    #pragma unroll
+                    for (int k = 0; k < PW; ++k)
+                        if (k < jb) col[j0 + k] = x[k];
                 }
             }
-            if (c >= j0 + jb) {
-                double x[PW];
-#pragma unroll
-                for (int k = 0; k < PW; ++k) x[k] = (k < jb) ? col[j0 + k] : 0.0;
-#pragma unroll
-                for (int k = 0; k < PW; ++k) {
-                    const double u = x[k];
-#pragma unroll
-                    for (int i = k + 1; i < PW; ++i)
-                        if (i < jb && (!exact || u != 0.0)) x[i] = fma(L11[i][k], -u, x[i]);
-                }
-#pragma unroll
-                for (int k = 0; k < PW; ++k)
-                    if (k < jb) col[j0 + k] = x[k];
-            }
+            __syncthreads();
+            if (j0 + jb < m && j0 + jb < n)
+                trailing_update<false>(A, lda, j0 + jb, m, j0 + jb, n, j0, jb, exact, Ls, Us);
         }
+        if (tid == 0) s.info[id] = info;
         __syncthreads();
-        if (j0 + jb < m && j0 + jb < n)
-            trailing_update<false>(A, lda, j0 + jb, m, j0 + jb, n, j0, jb, exact, Ls, Us);
     }
-    if (tid == 0) s.info[id] = info;
 }
 
 // ---------------------------------------------------------------------------
@@ -271,7 +290,7 @@ __global__ void __launch_bounds__(kThreads) getrf_blocked_kernel(MatSet s)
 // j0..n-1 of columns j0..j0+jb-1; the diagonal block's upper part is never
 // written back.
 template <int RPT, int PW>
-__global__ void __launch_bounds__(kThreads) potrf_blocked_kernel(MatSet s)
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) potrf_blocked_kernel(MatSet s)
 {
     __shared__ double lcol[PW];
     __shared__ double diag;
@@ -279,81 +298,85 @@ __global__ void __launch_bounds__(kThreads) potrf_blocked_kernel(MatSet s)
     __shared__ double Us[16][kTile + 1];
 
     const int tid = threadIdx.x;
-    const int item = blockIdx.x;
-    const int id = s.index ? s.index[item] : item;
-    const int n = pick(s.n_arr, s.n, id);
-    const int lda = pick(s.lda_arr, s.lda, id);
-    double *A = s.A[id];
-    int info = 0;
+    const int cnt = item_count(s);
+    const int *idx = item_index(s);
+    for (int item = blockIdx.x; item < cnt; item += gridDim.x) {
+        const int id = idx ? idx[item] : item;
+        const int n = pick(s.n_arr, s.n, id);
+        const int lda = pick(s.lda_arr, s.lda, id);
+        double *A = s.A[id];
+        int info = 0;
 
-    for (int j0 = 0; j0 < n && info == 0; j0 += PW) {
-        const int jb = min(PW, n - j0);
-        const int mp = n - j0;
-        double a[RPT][PW];
-#pragma unroll
-        for (int i = 0; i < RPT; ++i) {
-            const int pr = tid + kThreads * i;
-#pragma unroll
-            for (int c = 0; c < PW; ++c)
-                a[i][c] = (pr < mp && c < jb && c <= pr) ? A[(j0 + pr) + (size_t)(j0 + c) * lda] : 0.0;
-        }
-        bool bad = false;
-#pragma unroll
-        for (int jj = 0; jj < PW; ++jj) {
-            if (jj < jb && info == 0) {
-                if (tid == jj) diag = a[0][jj];
-                __syncthreads();
-                const double ajj = diag;
-                if (!(ajj > 0.0)) {
-                    info = j0 + jj + 1;
-                } else {
-                    const double ljj = sqrt(ajj);
-                    const double rc = 1.0 / ljj;
-                    if (tid == jj) a[0][jj] = ljj;
-                    double l[RPT];
-#pragma unroll
-                    for (int i = 0; i < RPT; ++i) {
-                        const int pr = tid + kThreads * i;
-                        l[i] = 0.0;
-                        if (pr < mp && pr > jj) {
-                            l[i] = rc * a[i][jj];
-                            a[i][jj] = l[i];
-                            bad |= !__builtin_isfinite(l[i]);
-                            if (pr < jb) lcol[pr] = l[i];
-                        }
-                    }
+        for (int j0 = 0; j0 < n && info == 0; j0 += PW) {
+            const int jb = min(PW, n - j0);
+            const int mp = n - j0;
+            double a[RPT][PW];
+    #pragma unroll
+            for (int i = 0; i < RPT; ++i) {
+                const int pr = tid + kThreads * i;
+    #pragma unroll
+                for (int c = 0; c < PW; ++c)
+                    a[i][c] = (pr < mp && c < jb && c <= pr) ? A[(j0 + pr) + (size_t)(j0 + c) * lda] : 0.0;
+            }
+            bool bad = false;
+    #pragma unroll
+            for (int jj = 0; jj < PW; ++jj) {
+                if (jj < jb && info == 0) {
+                    if (tid == jj) diag = a[0][jj];
                     __syncthreads();
-#pragma unroll
-                    for (int i = 0; i < RPT; ++i) {
-                        const int pr = tid + kThreads * i;
-                        if (pr < mp && pr > jj) {
-#pragma unroll
-                            for (int c = jj + 1; c < PW; ++c) {
-                                if (c < jb && c <= pr) {
-                                    const double u = lcol[c];
-                                    if (u != 0.0) a[i][c] = fma(-u, l[i], a[i][c]);
+                    const double ajj = diag;
+                    if (!(ajj > 0.0)) {
+                        info = j0 + jj + 1;
+                    } else {
+                        const double ljj = sqrt(ajj);
+                        const double rc = 1.0 / ljj;
+                        if (tid == jj) a[0][jj] = ljj;
+                        double l[RPT];
+    #pragma unroll
+                        for (int i = 0; i < RPT; ++i) {
+                            const int pr = tid + kThreads * i;
+                            l[i] = 0.0;
+                            if (pr < mp && pr > jj) {
+                                l[i] = rc * a[i][jj];
+                                a[i][jj] = l[i];
+                                bad |= !__builtin_isfinite(l[i]);
+                                if (pr < jb) lcol[pr] = l[i];
+                            }
+                        }
+                        __syncthreads();
+    #pragma unroll
+                        for (int i = 0; i < RPT; ++i) {
+                            const int pr = tid + kThreads * i;
+                            if (pr < mp && pr > jj) {
+    #pragma unroll
+                                for (int c = jj + 1; c < PW; ++c) {
+                                    if (c < jb && c <= pr) {
+                                        const double u = lcol[c];
+                                        if (u != 0.0) a[i][c] = fma(-u, l[i], a[i][c]);
+                                    }
                                 }
                             }
                         }
                     }
+                    __syncthreads();
                 }
-                __syncthreads();
             }
-        }
-#pragma unroll
-        for (int i = 0; i < RPT; ++i) {
-            const int pr = tid + kThreads * i;
-            if (pr < mp) {
-#pragma unroll
-                for (int c = 0; c < PW; ++c)
-                    if (c < jb && c <= pr) A[(j0 + pr) + (size_t)(j0 + c) * lda] = a[i][c];
+    #pragma unroll
+            for (int i = 0; i < RPT; ++i) {
+                const int pr = tid + kThreads * i;
+                if (pr < mp) {
+    #pragma unroll
+                    for (int c = 0; c < PW; ++c)
+                        if (c < jb && c <= pr) A[(j0 + pr) + (size_t)(j0 + c) * lda] = a[i][c];
+                }
             }
+            const bool exact = __syncthreads_or(bad);
+            if (info == 0 && j0 + jb < n)
+                trailing_update<true>(A, lda, j0 + jb, n, j0 + jb, n, j0, jb, exact, Ls, Us);
         }
-        const bool exact = __syncthreads_or(bad);
-        if (info == 0 && j0 + jb < n)
-            trailing_update<true>(A, lda, j0 + jb, n, j0 + jb, n, j0, jb, exact, Ls, Us);
+        if (tid == 0) s.info[id] = info;
+        __syncthreads();
     }
-    if (tid == 0) s.info[id] = info;
 }
 
 }  // namespace
@@ -361,10 +384,11 @@ __global__ void __launch_bounds__(kThreads) potrf_blocked_kernel(MatSet s)
 magma_int_t launch_getrf_blocked(const MatSet &s, int max_m, hipStream_t st)
 {
     if (s.count <= 0) return MAGMA_SUCCESS;
-    if (max_m <= 256)       getrf_blocked_kernel<1, 16><<<s.count, kThreads, 0, st>>>(s);
-    else if (max_m <= 512)  getrf_blocked_kernel<2, 16><<<s.count, kThreads, 0, st>>>(s);
-    else if (max_m <= 1024) getrf_blocked_kernel<4, 8><<<s.count, kThreads, 0, st>>>(s);
-    else if (max_m <= 2048) getrf_blocked_kernel<8, 4><<<s.count, kThreads, 0, st>>>(s);
+    const int grid = s.dev_count ? persistent_grid(s.count, 4) : s.count;
+    if (max_m <= 256)       getrf_blocked_kernel<1, 16><<<grid, kThreads, 0, st>>>(s);
+    else if (max_m <= 512)  getrf_blocked_kernel<2, 16><<<grid, kThreads, 0, st>>>(s);
+    else if (max_m <= 1024) getrf_blocked_kernel<4, 8><<<grid, kThreads, 0, st>>>(s);
+    else if (max_m <= 2048) getrf_blocked_kernel<8, 4><<<grid, kThreads, 0, st>>>(s);
     else return MAGMA_ERR_NOT_SUPPORTED;
     MAGMA_LAUNCH_CHECK();
     return MAGMA_SUCCESS;
@@ -373,10 +397,11 @@ magma_int_t launch_getrf_blocked(const MatSet &s, int max_m, hipStream_t st)
 magma_int_t launch_potrf_blocked(const MatSet &s, int max_n, hipStream_t st)
 {
     if (s.count <= 0) return MAGMA_SUCCESS;
-    if (max_n <= 256)       potrf_blocked_kernel<1, 16><<<s.count, kThreads, 0, st>>>(s);
-    else if (max_n <= 512)  potrf_blocked_kernel<2, 16><<<s.count, kThreads, 0, st>>>(s);
-    else if (max_n <= 1024) potrf_blocked_kernel<4, 8><<<s.count, kThreads, 0, st>>>(s);
-    else if (max_n <= 2048) potrf_blocked_kernel<8, 4><<<s.count, kThreads, 0, st>>>(s);
+    const int grid = s.dev_count ? persistent_grid(s.count, 4) : s.count;
+    if (max_n <= 256)       potrf_blocked_kernel<1, 16><<<grid, kThreads, 0, st>>>(s);
+    else if (max_n <= 512)  potrf_blocked_kernel<2, 16><<<grid, kThreads, 0, st>>>(s);
+    else if (max_n <= 1024) potrf_blocked_kernel<4, 8><<<grid, kThreads, 0, st>>>(s);
+    else if (max_n <= 2048) potrf_blocked_kernel<8, 4><<<grid, kThreads, 0, st>>>(s);
     else return MAGMA_ERR_NOT_SUPPORTED;
     MAGMA_LAUNCH_CHECK();
     return MAGMA_SUCCESS;

@@ -9,8 +9,9 @@
 
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 
-#include "internal.h"
+#include "kernels.h"
 
 namespace magma_amd {
 
@@ -55,6 +56,21 @@ void *queue_host(magma_queue_t queue, size_t bytes)
     }
     queue->host_bytes = want;
     return queue->host;
+}
+
+int persistent_grid(long items, int per_cu)
+{
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
+            cus = n;
+        else
+            cus = 256;
+    }
+    const long cap = (long)cus * per_cu;
+    return (int)std::max(1L, std::min(items, cap));
 }
 
 }  // namespace magma_amd
@@ -244,6 +260,8 @@ void magma_queue_sync_internal(magma_queue_t queue, const char *func, const char
 }
 
 magma_int_t magma_queue_get_device(magma_queue_t queue) { return queue->device; }
+
+void magma_queue_set_async(magma_queue_t queue, magma_int_t async) { queue->async = (async != 0); }
 
 void magma_event_create(magma_event_t *event_ptr)
 {

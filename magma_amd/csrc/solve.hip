@@ -10,6 +10,8 @@
 // dtrsm loop order, so results are bit-identical to LAPACK's dgetrs /
 // dpotrs forward sweep; the transposed sweeps associate their sums in the
 // opposite order to dtrsm's dot-product form and agree to rounding.
+#include <algorithm>
+
 #include "kernels.h"
 
 namespace magma_amd {
@@ -56,70 +58,74 @@ __device__ void sweep(const double *T, int ldt, int n, double *X, int ldx, int n
 }
 
 // mode: 'N' getrs no-trans, 'T' getrs trans, 'L' potrs lower, 'U' potrs upper.
-__global__ void __launch_bounds__(kThreads) solve_kernel(MatSet s, char mode)
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) solve_kernel(MatSet s, char mode, int chunk)
 {
     extern __shared__ double X[];
-    const int item = blockIdx.x;
-    const int id = s.index ? s.index[item] : item;
-    const int n = pick(s.n_arr, s.n, id);
-    const int lda = pick(s.lda_arr, s.lda, id);
-    const int nrhs = pick(s.nrhs_arr, s.nrhs, id);
-    const int ldb = pick(s.ldb_arr, s.ldb, id);
-    const double *A = s.A[id];
-    double *B = s.B[id];
-    const int *ipiv = (mode == 'N' || mode == 'T') ? s.ipiv[id] : nullptr;
-    const int tid = threadIdx.x;
-    const int ldx = n;
-    if (n <= 0) return;
-    // after a fused factorization: no solve with a singular factor (dgesv/dposv)
-    if (s.info && s.info[id] != 0) return;
+    const int cnt = item_count(s);
+    const int *idx = item_index(s);
+    for (int item = blockIdx.x; item < cnt; item += gridDim.x) {
+        const int id = idx ? idx[item] : item;
+        const int n = pick(s.n_arr, s.n, id);
+        const int lda = pick(s.lda_arr, s.lda, id);
+        const int nrhs = pick(s.nrhs_arr, s.nrhs, id);
+        const int ldb = pick(s.ldb_arr, s.ldb, id);
+        const double *A = s.A[id];
+        double *B = s.B[id];
+        const int *ipiv = (mode == 'N' || mode == 'T') ? s.ipiv[id] : nullptr;
+        const int tid = threadIdx.x;
+        const int ldx = n;
+        if (n <= 0) continue;
+        // after a fused factorization: no solve with a singular factor (dgesv/dposv)
+        if (s.info && s.info[id] != 0) continue;
 
-    for (int c0 = 0; c0 < nrhs; c0 += kChunk) {
-        const int nc = min(kChunk, nrhs - c0);
-        for (int e = tid; e < n * nc; e += kThreads) {
-            const int i = e % n, c = e / n;
-            X[(size_t)c * ldx + i] = B[i + (size_t)(c0 + c) * ldb];
-        }
-        __syncthreads();
-        if (mode == 'N') {
-            // dlaswp forward: interchanges in order, one thread per column
-            if (tid < nc)
-                for (int k = 0; k < n; ++k) {
-                    const int p = ipiv[k] - 1;
-                    if (p != k && p >= 0 && p < n) {
-                        double *x = X + (size_t)tid * ldx;
-                        const double t = x[k];
-                        x[k] = x[p];
-                        x[p] = t;
-                    }
-                }
+        for (int c0 = 0; c0 < nrhs; c0 += chunk) {
+            const int nc = min(chunk, nrhs - c0);
+            for (int e = tid; e < n * nc; e += kThreads) {
+                const int i = e % n, c = e / n;
+                X[(size_t)c * ldx + i] = B[i + (size_t)(c0 + c) * ldb];
+            }
             __syncthreads();
-            sweep(A, lda, n, X, ldx, nc, true, false, true);
-            sweep(A, lda, n, X, ldx, nc, false, false, false);
-        } else if (mode == 'T') {
-            sweep(A, lda, n, X, ldx, nc, false, true, false);
-            sweep(A, lda, n, X, ldx, nc, true, true, true);
-            if (tid < nc)
-                for (int k = n - 1; k >= 0; --k) {
-                    const int p = ipiv[k] - 1;
-                    if (p != k && p >= 0 && p < n) {
-                        double *x = X + (size_t)tid * ldx;
-                        const double t = x[k];
-                        x[k] = x[p];
-                        x[p] = t;
+            if (mode == 'N') {
+                // dlaswp forward: interchanges in order, one thread per column
+                if (tid < nc)
+                    for (int k = 0; k < n; ++k) {
+                        const int p = ipiv[k] - 1;
+                        if (p != k && p >= 0 && p < n) {
+                            double *x = X + (size_t)tid * ldx;
+                            const double t = x[k];
+                            x[k] = x[p];
+                            x[p] = t;
+                        }
                     }
-                }
+                __syncthreads();
+                sweep(A, lda, n, X, ldx, nc, true, false, true);
+                sweep(A, lda, n, X, ldx, nc, false, false, false);
+            } else if (mode == 'T') {
+                sweep(A, lda, n, X, ldx, nc, false, true, false);
+                sweep(A, lda, n, X, ldx, nc, true, true, true);
+                if (tid < nc)
+                    for (int k = n - 1; k >= 0; --k) {
+                        const int p = ipiv[k] - 1;
+                        if (p != k && p >= 0 && p < n) {
+                            double *x = X + (size_t)tid * ldx;
+                            const double t = x[k];
+                            x[k] = x[p];
+                            x[p] = t;
+                        }
+                    }
+                __syncthreads();
+            } else if (mode == 'L') {
+                sweep(A, lda, n, X, ldx, nc, true, false, false);
+                sweep(A, lda, n, X, ldx, nc, true, true, false);
+            } else {
+                sweep(A, lda, n, X, ldx, nc, false, true, false);
+                sweep(A, lda, n, X, ldx, nc, false, false, false);
+            }
+            for (int e = tid; e < n * nc; e += kThreads) {
+                const int i = e % n, c = e / n;
+                B[i + (size_t)(c0 + c) * ldb] = X[(size_t)c * ldx + i];
+            }
             __syncthreads();
-        } else if (mode == 'L') {
-            sweep(A, lda, n, X, ldx, nc, true, false, false);
-            sweep(A, lda, n, X, ldx, nc, true, true, false);
-        } else {
-            sweep(A, lda, n, X, ldx, nc, false, true, false);
-            sweep(A, lda, n, X, ldx, nc, false, false, false);
-        }
-        for (int e = tid; e < n * nc; e += kThreads) {
-            const int i = e % n, c = e / n;
-            B[i + (size_t)(c0 + c) * ldb] = X[(size_t)c * ldx + i];
         }
         __syncthreads();
     }
@@ -130,9 +136,12 @@ __global__ void __launch_bounds__(kThreads) solve_kernel(MatSet s, char mode)
 static magma_int_t launch_solve(const MatSet &s, char mode, int max_n, hipStream_t st)
 {
     if (s.count <= 0 || max_n <= 0) return MAGMA_SUCCESS;
-    const size_t lds = sizeof(double) * (size_t)max_n * kChunk;
-    if (lds > 64 * 1024) return MAGMA_ERR_NOT_SUPPORTED;
-    solve_kernel<<<s.count, kThreads, lds, st>>>(s, mode);
+    // right-hand-side columns per pass: as many as fit 64 KB of LDS, at most kChunk
+    const int chunk = std::min<long>(kChunk, (64 * 1024) / (sizeof(double) * (size_t)max_n));
+    if (chunk < 1) return MAGMA_ERR_NOT_SUPPORTED;
+    const size_t lds = sizeof(double) * (size_t)max_n * chunk;
+    const int grid = s.dev_count ? persistent_grid(s.count, 4) : s.count;
+    solve_kernel<<<grid, kThreads, lds, st>>>(s, mode, chunk);
     MAGMA_LAUNCH_CHECK();
     return MAGMA_SUCCESS;
 }

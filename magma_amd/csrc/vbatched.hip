@@ -1,0 +1,507 @@
+// vbatched.hip -- variable-size entry points and the batched GEMM.
+//
+// A vbatched call bins the matrices by size class on the device
+// (workqueue.hip) and runs each class on the kernel family built for it:
+// register-resident kernels up to 64, blocked workgroup-per-matrix kernels
+// above.  Two modes, as in the reference's API:
+//   * checked entry points (magma_dgetrf_vbatched, ...) validate the size
+//     arrays with a device kernel and read the flags and per-class counts back
+//     in one synchronizing copy (the reference reads back its checker flags
+//     and max sizes the same way, magmablas/vbatched_check.cu:48-105,
+//     src/zpotrf_vbatched.cpp:343-351); each non-empty class is then launched
+//     with its exact count;
+//   * *_max_nocheck entry points skip validation and never synchronize: the
+//     class kernels are persistent and read their counts on the device, so the
+//     whole call can be captured in a HIP graph.
+#include <algorithm>
+
+#include "kernels.h"
+#include "workqueue.h"
+
+namespace magma_amd {
+
+magma_int_t launch_getrf_small_v(const MatSet &s, int max_mn_dim, bool solve, bool var, hipStream_t st);
+magma_int_t launch_potrf_small_v(const MatSet &s, int max_n, bool solve, bool var, hipStream_t st);
+magma_int_t launch_getrf_blocked(const MatSet &s, int max_m, hipStream_t st);
+magma_int_t launch_potrf_blocked(const MatSet &s, int max_n, hipStream_t st);
+magma_int_t launch_gemm_tiles(const GemmSet &g, const int *prefix, long max_tiles, hipStream_t st);
+magma_int_t zero_info(int *info, int count, hipStream_t st);
+
+namespace {
+
+// flag slots written by the checkers
+enum { kFlagM = 0, kFlagN, kFlagK, kFlagLda, kFlagLdb, kFlagLdc, kFlagNrhs, kFlagTooBig, kNumFlags };
+
+__global__ void clear_kernel(int *p, int count)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = 0;
+}
+
+// Validates per-matrix sizes.  mode: 'F' factorization (m, n, lda >= max(1,m)),
+// 'C' Cholesky (n, lda >= n, reference rule), 'S' solve (n, nrhs, lda, ldb),
+// 'G' gemm (m, n, k, lda, ldb, ldc for the given transposes).
+__global__ void check_kernel(char mode, char ta, char tb, const int *m, const int *n, const int *k,
+                             const int *lda, const int *ldb, const int *ldc, int count, int limit,
+                             int *flags)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const int ni = n[i];
+    const int mi = m ? m[i] : ni;
+    if (mode == 'F') {
+        if (mi < 0) flags[kFlagM] = 1;
+        if (ni < 0) flags[kFlagN] = 1;
+        if (lda[i] < max(1, mi)) flags[kFlagLda] = 1;
+        if (max(mi, ni) > limit) flags[kFlagTooBig] = 1;
+    } else if (mode == 'C') {
+        if (ni < 0) flags[kFlagN] = 1;
+        if (lda[i] < ni) flags[kFlagLda] = 1;
+        if (ni > limit) flags[kFlagTooBig] = 1;
+    } else if (mode == 'S') {
+        if (ni < 0) flags[kFlagN] = 1;
+        if (k[i] < 0) flags[kFlagNrhs] = 1;
+        if (lda[i] < max(1, ni)) flags[kFlagLda] = 1;
+        if (ldb[i] < max(1, ni)) flags[kFlagLdb] = 1;
+        if (ni > limit) flags[kFlagTooBig] = 1;
+    } else {
+        const int ki = k[i];
+        if (mi < 0) flags[kFlagM] = 1;
+        if (ni < 0) flags[kFlagN] = 1;
+        if (ki < 0) flags[kFlagK] = 1;
+        if (ta == 'N' ? lda[i] < mi : lda[i] < ki) flags[kFlagLda] = 1;
+        if (tb == 'N' ? ldb[i] < ki : ldb[i] < ni) flags[kFlagLdb] = 1;
+        if (ldc[i] < mi) flags[kFlagLdc] = 1;
+    }
+}
+
+struct VScratch {
+    int *flags = nullptr;
+    WorkQueue q;
+    int *prefix = nullptr;      // gemm tile prefix [count + 1]
+    int *host = nullptr;        // pinned: flags + counts
+};
+
+// Carves the queue workspace into checker flags, work queue and tile prefix.
+bool vscratch(magma_queue_t queue, int count, VScratch &v)
+{
+    const size_t fl = round_up(sizeof(int) * kNumFlags, 256);
+    const size_t wq = round_up(workqueue_bytes(count), 256);
+    const size_t pf = round_up(sizeof(int) * (size_t)(count + 1), 256);
+    char *base = static_cast<char *>(queue_workspace(queue, fl + wq + pf));
+    if (!base) return false;
+    v.flags = reinterpret_cast<int *>(base);
+    v.q = workqueue_view(base + fl, count);
+    v.prefix = reinterpret_cast<int *>(base + fl + wq);
+    v.host = static_cast<int *>(queue_host(queue, sizeof(int) * (kNumFlags + 2 * kNumBuckets)));
+    return v.host != nullptr;
+}
+
+// Runs the checker, builds the work queue and reads flags + class counts back.
+magma_int_t check_and_bucket(magma_queue_t queue, VScratch &v, char mode, char ta, char tb,
+                             const int *m, const int *n, const int *k, const int *lda, const int *ldb,
+                             const int *ldc, int count, int limit, bool bucket)
+{
+    hipStream_t st = stream_of(queue);
+    clear_kernel<<<1, 64, 0, st>>>(v.flags, kNumFlags);
+    check_kernel<<<ceil_div(count, 256), 256, 0, st>>>(mode, ta, tb, m, n, k, lda, ldb, ldc, count, limit,
+                                                       v.flags);
+    MAGMA_LAUNCH_CHECK();
+    if (bucket) {
+        magma_int_t r = build_workqueue(m, n, count, v.q, st);
+        if (r != MAGMA_SUCCESS) return r;
+    }
+    MAGMA_HIP_CHECK(hipMemcpyAsync(v.host, v.flags, sizeof(int) * kNumFlags, hipMemcpyDeviceToHost, st));
+    if (bucket) {
+        MAGMA_HIP_CHECK(hipMemcpyAsync(v.host + kNumFlags, v.q.counts, sizeof(int) * kNumBuckets,
+                                       hipMemcpyDeviceToHost, st));
+        MAGMA_HIP_CHECK(hipMemcpyAsync(v.host + kNumFlags + kNumBuckets, v.q.offsets, sizeof(int) * kNumBuckets,
+                                       hipMemcpyDeviceToHost, st));
+    }
+    MAGMA_HIP_CHECK(hipStreamSynchronize(st));
+    return MAGMA_SUCCESS;
+}
+
+magma_int_t finish(magma_int_t r, magma_queue_t queue)
+{
+    if (r == MAGMA_SUCCESS && !(queue && queue->async)) {
+        hipError_t e = hipStreamSynchronize(stream_of(queue));
+        if (e != hipSuccess) return report_hip_error(e, "hipStreamSynchronize", __FILE__, __LINE__);
+    }
+    return r;
+}
+
+// Launches one size class.  kind: 'L' getrf, 'C' potrf, 'N'/'T' getrs, 'l'/'u' potrs.
+magma_int_t launch_class(char kind, int b, MatSet s, hipStream_t st)
+{
+    const int lim = kBucketLimit[b];
+    switch (kind) {
+    case 'L': return lim <= 64 ? launch_getrf_small_v(s, lim, false, true, st) : launch_getrf_blocked(s, lim, st);
+    case 'C': return lim <= 64 ? launch_potrf_small_v(s, lim, false, true, st) : launch_potrf_blocked(s, lim, st);
+    case 'N':
+    case 'T': return launch_getrs(s, kind, lim, 0, st);
+    case 'l': return launch_potrs(s, 'L', lim, 0, st);
+    default:  return launch_potrs(s, 'U', lim, 0, st);
+    }
+}
+
+// Checked path: exact per-class counts from the host copy.
+magma_int_t run_classes(char kind, MatSet s, const VScratch &v, hipStream_t st)
+{
+    const int *counts = v.host + kNumFlags;
+    const int *offsets = v.host + kNumFlags + kNumBuckets;
+    for (int b = 0; b < kNumBuckets; ++b) {
+        if (counts[b] == 0) continue;
+        MatSet c = s;
+        c.index = v.q.ids + offsets[b];
+        c.count = counts[b];
+        magma_int_t r = launch_class(kind, b, c, st);
+        if (r != MAGMA_SUCCESS) return r;
+    }
+    return MAGMA_SUCCESS;
+}
+
+// Unchecked path: persistent class kernels reading their counts on the device,
+// for every class up to the one holding max_dim.
+magma_int_t run_classes_device(char kind, MatSet s, const VScratch &v, int max_dim, hipStream_t st)
+{
+    for (int b = 0; b < kNumBuckets; ++b) {
+        if (b > 0 && kBucketLimit[b - 1] >= max_dim) break;
+        MatSet c = s;
+        c.index = v.q.ids;
+        c.dev_offset = v.q.offsets + b;
+        c.dev_count = v.q.counts + b;
+        magma_int_t r = launch_class(kind, b, c, st);
+        if (r != MAGMA_SUCCESS) return r;
+    }
+    return MAGMA_SUCCESS;
+}
+
+constexpr int kMaxFactorDim = 2048;
+
+}  // namespace
+}  // namespace magma_amd
+
+using namespace magma_amd;
+
+extern "C" {
+
+// ---------------------------------------------------------------- LU -------
+magma_int_t magma_dgetrf_vbatched(magma_int_t *m, magma_int_t *n, double **dA_array, magma_int_t *ldda,
+                                  magma_int_t **dipiv_array, magma_int_t *info_array,
+                                  magma_int_t batchCount, magma_queue_t queue)
+{
+    if (batchCount < 0) {
+        magma_xerbla(__func__, 8);
+        return -8;
+    }
+    if (batchCount == 0) return MAGMA_SUCCESS;
+    VScratch v;
+    if (!vscratch(queue, batchCount, v)) return MAGMA_ERR_DEVICE_ALLOC;
+    magma_int_t r = check_and_bucket(queue, v, 'F', 'N', 'N', m, n, nullptr, ldda, nullptr, nullptr,
+                                     batchCount, kMaxFactorDim, true);
+    if (r != MAGMA_SUCCESS) return r;
+    const int *f = v.host;
+    magma_int_t arg = 0;
+    if (f[kFlagM]) arg = -1;
+    else if (f[kFlagN]) arg = -2;
+    else if (f[kFlagLda]) arg = -4;
+    if (arg != 0) {
+        zero_info(info_array, batchCount, stream_of(queue));
+        magma_xerbla(__func__, -arg);
+        return arg;
+    }
+    if (f[kFlagTooBig]) return MAGMA_ERR_NOT_SUPPORTED;
+    MatSet s;
+    s.m_arr = m; s.n_arr = n; s.lda_arr = ldda;
+    s.A = dA_array; s.ipiv = dipiv_array; s.info = info_array;
+    return finish(run_classes('L', s, v, stream_of(queue)), queue);
+}
+
+magma_int_t magma_dgetrf_vbatched_max_nocheck(magma_int_t *m, magma_int_t *n, magma_int_t max_mn,
+                                              double **dA_array, magma_int_t *ldda,
+                                              magma_int_t **dipiv_array, magma_int_t *info_array,
+                                              magma_int_t batchCount, magma_queue_t queue)
+{
+    if (batchCount <= 0) return MAGMA_SUCCESS;
+    if (max_mn > kMaxFactorDim) return MAGMA_ERR_NOT_SUPPORTED;
+    VScratch v;
+    if (!vscratch(queue, batchCount, v)) return MAGMA_ERR_DEVICE_ALLOC;
+    hipStream_t st = stream_of(queue);
+    magma_int_t r = build_workqueue(m, n, batchCount, v.q, st);
+    if (r != MAGMA_SUCCESS) return r;
+    MatSet s;
+    s.m_arr = m; s.n_arr = n; s.lda_arr = ldda;
+    s.A = dA_array; s.ipiv = dipiv_array; s.info = info_array;
+    s.count = batchCount;
+    return finish(run_classes_device('L', s, v, max_mn, st), queue);
+}
+
+magma_int_t magma_dgetrs_vbatched(magma_trans_t trans, magma_int_t *n, magma_int_t *nrhs,
+                                  double **dA_array, magma_int_t *ldda, magma_int_t **dipiv_array,
+                                  double **dB_array, magma_int_t *lddb,
+                                  magma_int_t batchCount, magma_queue_t queue)
+{
+    if (trans != MagmaNoTrans && trans != MagmaTrans && trans != MagmaConjTrans) {
+        magma_xerbla(__func__, 1);
+        return -1;
+    }
+    if (batchCount < 0) {
+        magma_xerbla(__func__, 10);
+        return -10;
+    }
+    if (batchCount == 0) return MAGMA_SUCCESS;
+    VScratch v;
+    if (!vscratch(queue, batchCount, v)) return MAGMA_ERR_DEVICE_ALLOC;
+    magma_int_t r = check_and_bucket(queue, v, 'S', 'N', 'N', nullptr, n, nrhs, ldda, lddb, nullptr,
+                                     batchCount, kMaxFactorDim, true);
+    if (r != MAGMA_SUCCESS) return r;
+    const int *f = v.host;
+    magma_int_t arg = 0;
+    if (f[kFlagN]) arg = -2;
+    else if (f[kFlagNrhs]) arg = -3;
+    else if (f[kFlagLda]) arg = -5;
+    else if (f[kFlagLdb]) arg = -8;
+    if (arg != 0) {
+        magma_xerbla(__func__, -arg);
+        return arg;
+    }
+    if (f[kFlagTooBig]) return MAGMA_ERR_NOT_SUPPORTED;
+    MatSet s;
+    s.n_arr = n; s.m_arr = n; s.lda_arr = ldda;
+    s.nrhs_arr = nrhs; s.ldb_arr = lddb;
+    s.A = dA_array; s.ipiv = dipiv_array; s.B = dB_array;
+    return finish(run_classes(trans == MagmaNoTrans ? 'N' : 'T', s, v, stream_of(queue)), queue);
+}
+
+// ---------------------------------------------------------------- Cholesky ---
+// src/zpotrf_vbatched.cpp:335-356; checks magmablas/vbatched_check.cu:840-892
+magma_int_t magma_dpotrf_vbatched(magma_uplo_t uplo, magma_int_t *n, double **dA_array, magma_int_t *ldda,
+                                  magma_int_t *info_array, magma_int_t batchCount, magma_queue_t queue)
+{
+    magma_int_t arg = 0;
+    if (uplo != MagmaLower && uplo != MagmaUpper) arg = -1;
+    else if (batchCount < 0) arg = -6;
+    if (arg == 0 && uplo == MagmaUpper) {
+        std::fprintf(stderr, "%s: uplo=upper is not yet implemented\n", __func__);
+        arg = -1;
+    }
+    if (arg != 0) {
+        magma_xerbla(__func__, -arg);
+        return arg;
+    }
+    if (batchCount == 0) return MAGMA_SUCCESS;
+    VScratch v;
+    if (!vscratch(queue, batchCount, v)) return MAGMA_ERR_DEVICE_ALLOC;
+    magma_int_t r = check_and_bucket(queue, v, 'C', 'N', 'N', nullptr, n, nullptr, ldda, nullptr, nullptr,
+                                     batchCount, kMaxFactorDim, true);
+    if (r != MAGMA_SUCCESS) return r;
+    const int *f = v.host;
+    if (f[kFlagN]) arg = -2;
+    else if (f[kFlagLda]) arg = -4;
+    if (arg != 0) {
+        magma_xerbla(__func__, -arg);
+        return arg;
+    }
+    if (f[kFlagTooBig]) return MAGMA_ERR_NOT_SUPPORTED;
+    MatSet s;
+    s.n_arr = n; s.m_arr = n; s.lda_arr = ldda;
+    s.A = dA_array; s.info = info_array;
+    return finish(run_classes('C', s, v, stream_of(queue)), queue);
+}
+
+magma_int_t magma_dpotrf_vbatched_max_nocheck(magma_uplo_t uplo, magma_int_t *n, double **dA_array,
+                                              magma_int_t *ldda, magma_int_t *info_array,
+                                              magma_int_t batchCount, magma_int_t max_n, magma_queue_t queue)
+{
+    if (uplo != MagmaLower) {
+        magma_xerbla(__func__, 1);
+        return -1;
+    }
+    if (batchCount <= 0) return MAGMA_SUCCESS;
+    if (max_n > kMaxFactorDim) return MAGMA_ERR_NOT_SUPPORTED;
+    VScratch v;
+    if (!vscratch(queue, batchCount, v)) return MAGMA_ERR_DEVICE_ALLOC;
+    hipStream_t st = stream_of(queue);
+    magma_int_t r = build_workqueue(nullptr, n, batchCount, v.q, st);
+    if (r != MAGMA_SUCCESS) return r;
+    MatSet s;
+    s.n_arr = n; s.m_arr = n; s.lda_arr = ldda;
+    s.A = dA_array; s.info = info_array;
+    s.count = batchCount;
+    return finish(run_classes_device('C', s, v, max_n, st), queue);
+}
+
+magma_int_t magma_dpotrs_vbatched(magma_uplo_t uplo, magma_int_t *n, magma_int_t *nrhs,
+                                  double **dA_array, magma_int_t *ldda, double **dB_array, magma_int_t *lddb,
+                                  magma_int_t batchCount, magma_queue_t queue)
+{
+    if (uplo != MagmaLower && uplo != MagmaUpper) {
+        magma_xerbla(__func__, 1);
+        return -1;
+    }
+    if (batchCount < 0) {
+        magma_xerbla(__func__, 9);
+        return -9;
+    }
+    if (batchCount == 0) return MAGMA_SUCCESS;
+    VScratch v;
+    if (!vscratch(queue, batchCount, v)) return MAGMA_ERR_DEVICE_ALLOC;
+    magma_int_t r = check_and_bucket(queue, v, 'S', 'N', 'N', nullptr, n, nrhs, ldda, lddb, nullptr,
+                                     batchCount, kMaxFactorDim, true);
+    if (r != MAGMA_SUCCESS) return r;
+    const int *f = v.host;
+    magma_int_t arg = 0;
+    if (f[kFlagN]) arg = -2;
+    else if (f[kFlagNrhs]) arg = -3;
+    else if (f[kFlagLda]) arg = -5;
+    else if (f[kFlagLdb]) arg = -7;
+    if (arg != 0) {
+        magma_xerbla(__func__, -arg);
+        return arg;
+    }
+    if (f[kFlagTooBig]) return MAGMA_ERR_NOT_SUPPORTED;
+    MatSet s;
+    s.n_arr = n; s.m_arr = n; s.lda_arr = ldda;
+    s.nrhs_arr = nrhs; s.ldb_arr = lddb;
+    s.A = dA_array; s.B = dB_array;
+    return finish(run_classes(uplo == MagmaLower ? 'l' : 'u', s, v, stream_of(queue)), queue);
+}
+
+// ---------------------------------------------------------------- GEMM -------
+static char tr(magma_trans_t t) { return t == MagmaNoTrans ? 'N' : 'T'; }
+
+// magmablas/dgemm_batched_core.cu:180-200
+void magmablas_dgemm_batched(magma_trans_t transA, magma_trans_t transB,
+                             magma_int_t m, magma_int_t n, magma_int_t k, double alpha,
+                             double const *const *dA_array, magma_int_t ldda,
+                             double const *const *dB_array, magma_int_t lddb, double beta,
+                             double **dC_array, magma_int_t lddc, magma_int_t batchCount, magma_queue_t queue)
+{
+    magma_int_t info = 0;
+    if (transA != MagmaNoTrans && transA != MagmaTrans && transA != MagmaConjTrans) info = -1;
+    else if (transB != MagmaNoTrans && transB != MagmaTrans && transB != MagmaConjTrans) info = -2;
+    else if (m < 0) info = -3;
+    else if (n < 0) info = -4;
+    else if (k < 0) info = -5;
+    else if (transA == MagmaNoTrans ? ldda < m : ldda < k) info = -8;
+    else if (transB == MagmaNoTrans ? lddb < k : lddb < n) info = -10;
+    else if (lddc < m) info = -13;
+    if (info != 0) {
+        magma_xerbla(__func__, -info);
+        return;
+    }
+    if (m == 0 || n == 0 || batchCount <= 0) return;
+    if ((k == 0 || alpha == 0.0) && beta == 1.0) return;
+    GemmSet g;
+    g.ta = tr(transA); g.tb = tr(transB);
+    g.m = m; g.n = n; g.k = k; g.lda = ldda; g.ldb = lddb; g.ldc = lddc;
+    g.alpha = alpha; g.beta = beta;
+    g.A = dA_array; g.B = dB_array; g.C = dC_array;
+    g.count = batchCount;
+    const long tiles = (long)ceil_div(m, 32) * ceil_div(n, 32) * batchCount;
+    launch_gemm_tiles(g, nullptr, tiles, stream_of(queue));
+}
+
+// magmablas/zgemm_batched.cpp:166-195 (no vendor-BLAS branch: one implementation)
+void magma_dgemm_batched(magma_trans_t transA, magma_trans_t transB,
+                         magma_int_t m, magma_int_t n, magma_int_t k, double alpha,
+                         double const *const *dA_array, magma_int_t ldda,
+                         double const *const *dB_array, magma_int_t lddb, double beta,
+                         double **dC_array, magma_int_t lddc, magma_int_t batchCount, magma_queue_t queue)
+{
+    magmablas_dgemm_batched(transA, transB, m, n, k, alpha, dA_array, ldda, dB_array, lddb, beta,
+                            dC_array, lddc, batchCount, queue);
+}
+
+void magmablas_dgemm_vbatched_max_nocheck(magma_trans_t transA, magma_trans_t transB,
+                                          magma_int_t *m, magma_int_t *n, magma_int_t *k, double alpha,
+                                          double const *const *dA_array, magma_int_t *ldda,
+                                          double const *const *dB_array, magma_int_t *lddb, double beta,
+                                          double **dC_array, magma_int_t *lddc, magma_int_t batchCount,
+                                          magma_int_t max_m, magma_int_t max_n, magma_int_t max_k,
+                                          magma_queue_t queue)
+{
+    (void)max_k;
+    if (batchCount <= 0 || max_m <= 0 || max_n <= 0) return;
+    VScratch v;
+    if (!vscratch(queue, batchCount, v)) {
+        std::fprintf(stderr, "%s: workspace allocation failed\n", __func__);
+        return;
+    }
+    hipStream_t st = stream_of(queue);
+    if (build_tile_prefix(m, n, batchCount, 32, 32, v.prefix, st) != MAGMA_SUCCESS) return;
+    GemmSet g;
+    g.ta = tr(transA); g.tb = tr(transB);
+    g.m_arr = m; g.n_arr = n; g.k_arr = k; g.lda_arr = ldda; g.ldb_arr = lddb; g.ldc_arr = lddc;
+    g.alpha = alpha; g.beta = beta;
+    g.A = dA_array; g.B = dB_array; g.C = dC_array;
+    g.count = batchCount;
+    const long tiles = (long)ceil_div(max_m, 32) * ceil_div(max_n, 32) * batchCount;
+    launch_gemm_tiles(g, v.prefix, tiles, st);
+}
+
+// magmablas/zgemm_vbatched.cpp:235-270
+void magmablas_dgemm_vbatched(magma_trans_t transA, magma_trans_t transB,
+                              magma_int_t *m, magma_int_t *n, magma_int_t *k, double alpha,
+                              double const *const *dA_array, magma_int_t *ldda,
+                              double const *const *dB_array, magma_int_t *lddb, double beta,
+                              double **dC_array, magma_int_t *lddc,
+                              magma_int_t batchCount, magma_queue_t queue)
+{
+    magma_int_t info = 0;
+    if (transA != MagmaNoTrans && transA != MagmaTrans && transA != MagmaConjTrans) info = -1;
+    else if (transB != MagmaNoTrans && transB != MagmaTrans && transB != MagmaConjTrans) info = -2;
+    else if (batchCount < 0) info = -14;
+    if (info == 0 && batchCount > 0) {
+        VScratch v;
+        if (!vscratch(queue, batchCount, v)) {
+            std::fprintf(stderr, "%s: workspace allocation failed\n", __func__);
+            return;
+        }
+        if (check_and_bucket(queue, v, 'G', tr(transA), tr(transB), m, n, k, ldda, lddb, lddc, batchCount,
+                             1 << 30, false) != MAGMA_SUCCESS)
+            return;
+        const int *f = v.host;
+        if (f[kFlagM]) info = -3;
+        else if (f[kFlagN]) info = -4;
+        else if (f[kFlagK]) info = -5;
+        else if (f[kFlagLda]) info = -8;
+        else if (f[kFlagLdb]) info = -10;
+        else if (f[kFlagLdc]) info = -13;
+    }
+    if (info != 0) {
+        magma_xerbla(__func__, -info);
+        return;
+    }
+    if (batchCount == 0) return;
+    // the tile list is exact (device prefix), so the max sizes only bound the grid
+    magmablas_dgemm_vbatched_max_nocheck(transA, transB, m, n, k, alpha, dA_array, ldda, dB_array, lddb,
+                                         beta, dC_array, lddc, batchCount, 1 << 12, 1 << 12, 1 << 12, queue);
+}
+
+void magmablas_dgemm_vbatched_max(magma_trans_t transA, magma_trans_t transB,
+                                  magma_int_t *m, magma_int_t *n, magma_int_t *k, double alpha,
+                                  double const *const *dA_array, magma_int_t *ldda,
+                                  double const *const *dB_array, magma_int_t *lddb, double beta,
+                                  double **dC_array, magma_int_t *lddc, magma_int_t batchCount,
+                                  magma_int_t max_m, magma_int_t max_n, magma_int_t max_k,
+                                  magma_queue_t queue)
+{
+    (void)max_m; (void)max_n; (void)max_k;
+    magmablas_dgemm_vbatched(transA, transB, m, n, k, alpha, dA_array, ldda, dB_array, lddb, beta,
+                             dC_array, lddc, batchCount, queue);
+}
+
+void magmablas_dgemm_vbatched_nocheck(magma_trans_t transA, magma_trans_t transB,
+                                      magma_int_t *m, magma_int_t *n, magma_int_t *k, double alpha,
+                                      double const *const *dA_array, magma_int_t *ldda,
+                                      double const *const *dB_array, magma_int_t *lddb, double beta,
+                                      double **dC_array, magma_int_t *lddc,
+                                      magma_int_t batchCount, magma_queue_t queue)
+{
+    magmablas_dgemm_vbatched_max_nocheck(transA, transB, m, n, k, alpha, dA_array, ldda, dB_array, lddb,
+                                         beta, dC_array, lddc, batchCount, 1 << 12, 1 << 12, 1 << 12, queue);
+}
+
+}  // extern "C"

@@ -121,3 +121,67 @@ def cond_batch(seed, batch, n, first=0):
     kappa, s = cond_batch_params(seed, batch, n, first)
     A = (U * s[:, None, :]) @ np.swapaxes(V, -1, -2)
     return to_colmajor(A), kappa
+
+
+# ---- the same generator on a torch device (large benchmark inputs) --------
+# int64 arithmetic wraps modulo 2**64 on every torch backend; right shifts of
+# negative int64 are arithmetic, so logical shifts mask the sign extension.
+def _t_lsr(x, k):
+    import torch
+    return (x >> k) & torch.tensor((1 << (64 - k)) - 1, dtype=torch.int64, device=x.device)
+
+
+def _t_u64(v):
+    """python int in [0, 2**64) as the int64 with the same bits"""
+    return v - (1 << 64) if v >= (1 << 63) else v
+
+
+def splitmix64_torch(x):
+    z = x + _t_u64(0x9E3779B97F4A7C15)
+    z = (z ^ _t_lsr(z, 30)) * _t_u64(0xBF58476D1CE4E5B9)
+    z = (z ^ _t_lsr(z, 27)) * _t_u64(0x94D049BB133111EB)
+    return z ^ _t_lsr(z, 31)
+
+
+def u01_torch(seed, mats, elems):
+    """u in [0,1): mats (B,1) and elems (1,E) int64 tensors -> (B,E) float64."""
+    import torch
+    key = (seed << 40) + (mats << 20) + elems
+    return _t_lsr(splitmix64_torch(key), 11).to(torch.float64) * 2.0 ** -53
+
+
+def uniform_pm1_torch(seed, first, batch, count, device):
+    import torch
+    mats = torch.arange(first, first + batch, dtype=torch.int64, device=device).view(-1, 1)
+    e = torch.arange(count, dtype=torch.int64, device=device).view(1, -1)
+    return 2.0 * u01_torch(seed, mats, e) - 1.0
+
+
+def normal_torch(seed, first, batch, count, device, elem0=0):
+    import torch
+    mats = torch.arange(first, first + batch, dtype=torch.int64, device=device).view(-1, 1)
+    e = torch.arange(elem0, elem0 + count, dtype=torch.int64, device=device).view(1, -1)
+    u1 = u01_torch(seed, mats, 2 * e)
+    u2 = u01_torch(seed, mats, 2 * e + 1)
+    return torch.sqrt(-2.0 * torch.log1p(-u1)) * torch.cos(2.0 * torch.pi * u2)
+
+
+def cond_batch_torch(seed, batch, n, device, first=0, chunk=1024):
+    """cfg4 matrices built on the device: (batch, n, n) column-major storage.
+    Same construction as cond_batch (QR of N(0,1) by the device's LAPACK)."""
+    import torch
+    out = torch.empty((batch, n, n), dtype=torch.float64, device=device)
+    kappa_all = []
+    for c0 in range(0, batch, chunk):
+        b = min(chunk, batch - c0)
+        G1 = normal_torch(seed, first + c0, b, n * n, device).view(b, n, n)       # storage (cols, rows)
+        G2 = normal_torch(seed, first + c0, b, n * n, device, elem0=n * n).view(b, n, n)
+        U, _ = torch.linalg.qr(G1.transpose(1, 2))
+        V, _ = torch.linalg.qr(G2.transpose(1, 2))
+        kappa, s = cond_batch_params(seed, b, n, first + c0)
+        st = torch.from_numpy(s).to(device)
+        A = torch.matmul(U * st[:, None, :], V.transpose(1, 2).contiguous())     # row-major A
+        out[c0:c0 + b] = A.transpose(1, 2)                                         # column-major storage
+        kappa_all.append(kappa)
+        del G1, G2, U, V, A
+    return out, np.concatenate(kappa_all)

@@ -54,6 +54,7 @@ PROTOTYPES = {
     "magma_queue_sync_internal": (None, [_P, c_char_p, c_char_p, c_int]),
     "magma_queue_get_device": (c_int, [_P]),
     "magma_queue_get_hip_stream": (_P, [_P]),
+    "magma_queue_set_async": (None, [_P, c_int]),
     "magma_event_create": (None, [_P]),
     "magma_event_destroy": (None, [_P]),
     "magma_event_record": (None, [_P, _P]),
@@ -73,6 +74,27 @@ PROTOTYPES = {
     "magma_dpotrf_batched": (c_int, [c_int, c_int, _P, c_int, _P, c_int, _P]),
     "magma_dpotrs_batched": (c_int, [c_int, c_int, c_int, _P, c_int, _P, c_int, c_int, _P]),
     "magma_dposv_batched": (c_int, [c_int, c_int, c_int, _P, c_int, _P, c_int, _P, c_int, _P]),
+    "magma_dgemm_batched": (None, [c_int, c_int, c_int, c_int, c_int, c_double, _P, c_int, _P, c_int,
+                                   c_double, _P, c_int, c_int, _P]),
+    "magmablas_dgemm_batched": (None, [c_int, c_int, c_int, c_int, c_int, c_double, _P, c_int, _P, c_int,
+                                       c_double, _P, c_int, c_int, _P]),
+    # mixed precision (include/magma_amd.h)
+    "magma_dsgesv_batched": (c_int, [c_int, c_int, _P, c_int, _P, _P, c_int, _P, c_int, _P, _P, c_int, _P]),
+    # vbatched (include/magma_dvbatched.h)
+    "magma_dgetrf_vbatched": (c_int, [_P, _P, _P, _P, _P, _P, c_int, _P]),
+    "magma_dgetrf_vbatched_max_nocheck": (c_int, [_P, _P, c_int, _P, _P, _P, _P, c_int, _P]),
+    "magma_dgetrs_vbatched": (c_int, [c_int, _P, _P, _P, _P, _P, _P, _P, c_int, _P]),
+    "magma_dpotrf_vbatched": (c_int, [c_int, _P, _P, _P, _P, c_int, _P]),
+    "magma_dpotrf_vbatched_max_nocheck": (c_int, [c_int, _P, _P, _P, _P, c_int, c_int, _P]),
+    "magma_dpotrs_vbatched": (c_int, [c_int, _P, _P, _P, _P, _P, _P, c_int, _P]),
+    "magmablas_dgemm_vbatched": (None, [c_int, c_int, _P, _P, _P, c_double, _P, _P, _P, _P, c_double,
+                                        _P, _P, c_int, _P]),
+    "magmablas_dgemm_vbatched_max_nocheck": (None, [c_int, c_int, _P, _P, _P, c_double, _P, _P, _P, _P,
+                                                    c_double, _P, _P, c_int, c_int, c_int, c_int, _P]),
+    "magmablas_dgemm_vbatched_max": (None, [c_int, c_int, _P, _P, _P, c_double, _P, _P, _P, _P, c_double,
+                                            _P, _P, c_int, c_int, c_int, c_int, _P]),
+    "magmablas_dgemm_vbatched_nocheck": (None, [c_int, c_int, _P, _P, _P, c_double, _P, _P, _P, _P,
+                                                c_double, _P, _P, c_int, _P]),
 }
 
 OPTIONAL = set()

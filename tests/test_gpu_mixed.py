@@ -1,0 +1,97 @@
+"""GPU: magma_dsgesv_batched (FP32 factorization with split-FP16 MFMA updates,
+FP64 refinement, FP64 fallback).
+
+Bar: every solution has backward error ||b - Ax||_inf / (||A||_inf ||x||_inf)
+<= n * eps64 (oracle-computed, orc_solve_error <= 1); matrices within the
+refinement's reach (cond <= 1e6) converge without fallback and leave A and B
+unchanged; singular, NaN and very ill-conditioned matrices take the FP64 path
+with dsgesv's iter codes and then meet the plain solver's bar (< 30).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def run_dsgesv(queue, A, B):
+    import magma_amd as ma
+    import torch
+    count, n, _ = A.shape
+    nrhs = B.shape[1]
+    dA = ma.DeviceBatch(A)
+    dB = ma.DeviceBatch(B)
+    dX = ma.DeviceBatch(np.zeros_like(B))
+    ipiv = ma.IntBatch(count, n)
+    info = torch.zeros(count, dtype=torch.int32, device="cuda")
+    it = torch.full((count,), 99, dtype=torch.int32, device="cuda")
+    print(f"dsgesv count={count} n={n} nrhs={nrhs}", flush=True)
+    ret = ma.load().magma_dsgesv_batched(n, nrhs, dA.ptr, dA.ld, ipiv.ptr, dB.ptr, dB.ld, dX.ptr, dX.ld,
+                                         info.data_ptr(), it.data_ptr(), count, queue.handle)
+    assert ret == 0
+    return dA.numpy(), dB.numpy(), dX.numpy(), ipiv.numpy(), info.cpu().numpy(), it.cpu().numpy()
+
+
+@pytest.mark.parametrize("n,nrhs,count", [(256, 1, 48), (100, 2, 40), (33, 1, 40), (300, 1, 12), (1, 1, 4),
+                                          (64, 5, 20)])
+def test_dsgesv_cond_matrices(queue, oracle, n, nrhs, count):
+    from magma_amd import gen
+    A, kappa = gen.cond_batch(3, count, n) if n > 1 else (np.full((count, 1, 1), 0.5), np.ones(count))
+    B = gen.rhs_batch(3, count, n, nrhs)
+    Ag, Bg, X, ipiv, info, it = run_dsgesv(queue, A, B)
+    assert np.all(info == 0)
+    assert np.all(it >= 0), f"fallbacks: {np.sum(it < 0)} iter={it}"
+    assert np.all(it <= 10)
+    assert np.array_equal(Ag, A) and np.array_equal(Bg, B)       # untouched when refinement succeeds
+    for s in range(count):
+        be = oracle.solve_error(A[s], X[s], B[s])                 # ||r|| / (n ||A|| ||x|| eps)
+        assert be <= 1.0, (s, be, kappa[s], it[s])
+
+
+def test_dsgesv_fallbacks(queue, oracle):
+    from magma_amd import gen
+    n, count = 96, 8
+    A, _ = gen.cond_batch(5, count, n)
+    A = A.copy()
+    A[1] = 0.0                                                    # zero matrix: singular (iter -3, info 1)
+    A[2][5, :] = A[2][4, :]                                       # singular: two equal columns
+    A[3][7, 9] = np.nan                                           # cannot be scaled (iter -2)
+    # condition 1e13: out of the refinement's reach (iter -11), FP64 still solves it
+    U, _ = np.linalg.qr(np.random.default_rng(0).standard_normal((n, n)))
+    V, _ = np.linalg.qr(np.random.default_rng(1).standard_normal((n, n)))
+    s = 10.0 ** (-13 * np.arange(n) / (n - 1))
+    A[4] = np.ascontiguousarray(((U * s) @ V.T).T)
+    B = gen.rhs_batch(5, count, n, 1)
+    Ag, Bg, X, ipiv, info, it = run_dsgesv(queue, A, B)
+    assert it[0] >= 0 and it[5] >= 0
+    assert it[1] == -3 and info[1] == 1
+    assert it[2] < 0                                              # singular: FP32 pivot may be tiny, not zero
+    assert it[3] == -2
+    assert it[4] == -(10 + 1)
+    # the fallback factor is LAPACK's: A holds the FP64 LU and ipiv its pivots
+    LUo, ipo, info_o = oracle.dgetrf(A[4])
+    assert np.array_equal(ipiv[4], ipo) and np.array_equal(Ag[4], LUo)
+    assert oracle.solve_error(A[4], X[4], B[4]) < 30
+    Xo = oracle.dgetrs(LUo, ipo, B[4])
+    assert np.array_equal(X[4], Xo)
+
+
+def test_dsgesv_large_n_falls_back(queue, oracle):
+    from magma_amd import gen
+    n, count = 600, 2
+    A = gen.general_batch(8, count, n)
+    B = gen.rhs_batch(8, count, n, 1)
+    Ag, Bg, X, ipiv, info, it = run_dsgesv(queue, A, B)
+    assert np.all(it == -1)
+    for s in range(count):
+        assert oracle.solve_error(A[s], X[s], B[s]) < 30
+
+
+def test_dsgesv_argument_errors(queue):
+    import magma_amd as ma
+    L = ma.load()
+    q = queue.handle
+    assert L.magma_dsgesv_batched(-1, 1, 0, 1, 0, 0, 1, 0, 1, 0, 0, 1, q) == -1
+    assert L.magma_dsgesv_batched(4, -1, 0, 4, 0, 0, 4, 0, 4, 0, 0, 1, q) == -2
+    assert L.magma_dsgesv_batched(4, 1, 0, 3, 0, 0, 4, 0, 4, 0, 0, 1, q) == -4
+    assert L.magma_dsgesv_batched(4, 1, 0, 4, 0, 0, 3, 0, 4, 0, 0, 1, q) == -7
+    assert L.magma_dsgesv_batched(4, 1, 0, 4, 0, 0, 4, 0, 3, 0, 0, 1, q) == -9

@@ -394,11 +394,10 @@ __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(2))) ds
             for (int c = 0; c < n; ++c) {
                 const double v = fabs(A[r + (size_t)c * lda]);
                 s += v;
-                m = fmax(m, v);
-                if (v != v) m = v;
+                m = nanmax(m, v);
             }
-            rowsum_max = (s > rowsum_max || s != s) ? s : rowsum_max;
-            amax = (m > amax || m != m) ? m : amax;
+            rowsum_max = nanmax(rowsum_max, s);
+            amax = nanmax(amax, m);
         }
         const double anrm = block_max(rowsum_max, sm.red);
         const double amx = block_max(amax, sm.red);

@@ -166,18 +166,33 @@ def normal_torch(seed, first, batch, count, device, elem0=0):
     return torch.sqrt(-2.0 * torch.log1p(-u1)) * torch.cos(2.0 * torch.pi * u2)
 
 
-def cond_batch_torch(seed, batch, n, device, first=0, chunk=1024):
+def _q_factor(G, state):
+    """Q of the QR of each matrix of G (rows, cols), on G's device when the
+    device solver accepts the batch, else on the host.  The batched device QR
+    can refuse large batches for want of solver workspace (a library status,
+    not a device error); the host QR then serves the remaining chunks."""
+    import torch
+    if state.get("device_qr", True):
+        try:
+            return torch.linalg.qr(G.contiguous())[0]
+        except RuntimeError:
+            state["device_qr"] = False
+    return torch.linalg.qr(G.cpu().contiguous())[0].to(G.device)
+
+
+def cond_batch_torch(seed, batch, n, device, first=0, chunk=128):
     """cfg4 matrices built on the device: (batch, n, n) column-major storage.
-    Same construction as cond_batch (QR of N(0,1) by the device's LAPACK)."""
+    Same construction as cond_batch (QR of N(0,1) matrices)."""
     import torch
     out = torch.empty((batch, n, n), dtype=torch.float64, device=device)
     kappa_all = []
+    state = {}
     for c0 in range(0, batch, chunk):
         b = min(chunk, batch - c0)
         G1 = normal_torch(seed, first + c0, b, n * n, device).view(b, n, n)       # storage (cols, rows)
         G2 = normal_torch(seed, first + c0, b, n * n, device, elem0=n * n).view(b, n, n)
-        U, _ = torch.linalg.qr(G1.transpose(1, 2))
-        V, _ = torch.linalg.qr(G2.transpose(1, 2))
+        U = _q_factor(G1.transpose(1, 2), state)
+        V = _q_factor(G2.transpose(1, 2), state)
         kappa, s = cond_batch_params(seed, b, n, first + c0)
         st = torch.from_numpy(s).to(device)
         A = torch.matmul(U * st[:, None, :], V.transpose(1, 2).contiguous())     # row-major A

@@ -84,6 +84,11 @@ class Timer:
         return [a.elapsed_time(b) for a, b in self.pairs]
 
 
+def log(msg):
+    """Progress line on stderr (stdout carries only the JSON result)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 def cpu_threads():
     try:
         return len(os.sched_getaffinity(0))
@@ -96,6 +101,7 @@ def run_fixed(args, torch, ma, L, q, make_inputs, step_fn, solves_per_step, kern
     """Generic driver: stage warmup+steps copies, time steps, return stats."""
     copies = [make_inputs(i) for i in range(args.warmup + args.steps)]
     torch.cuda.synchronize()
+    log(f"staged {len(copies)} input copies; warmup {args.warmup}, timed {args.steps}")
     for i in range(args.warmup):
         step_fn(copies[i], None)
     torch.cuda.synchronize()
@@ -106,6 +112,7 @@ def run_fixed(args, torch, ma, L, q, make_inputs, step_fn, solves_per_step, kern
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     step_ms = timer.ms()
+    log(f"timed region {wall * 1e3:.2f} ms")
     return wall, step_ms, copies
 
 
@@ -201,7 +208,9 @@ def workload_cond256(args, torch, ma, L, q):
     """cfg4: plain FP64 getrf+getrs vs mixed dsgesv on cond-controlled n=256."""
     from magma_amd import gen
     count, n, nrhs = args.batch or 8192, 256, 1
+    log(f"generating {count} conditioned matrices n={n}")
     A0, kappa = gen.cond_batch_torch(3, count, n, "cuda")
+    log("inputs ready")
     B0 = gen.uniform_pm1_torch(103, 0, count, n * nrhs, "cuda").view(count, nrhs, n).contiguous()
     ipiv = ma.IntBatch(count, n)
     info = torch.zeros(count, dtype=torch.int32, device="cuda")
@@ -231,7 +240,7 @@ def workload_cond256(args, torch, ma, L, q):
     res["plain"] = dict(solves_per_s=count / (plain_ms * 1e-3), ms_per_step=plain_ms, factor_kernel_ms=fact_ms)
     flops = count * getrf_flops(n)
     roof = {"bound": "mfma", "achieved": flops / (fact_ms * 1e-3) / 1e12, "peak": FP64_PEAK / 1e12,
-            "unit": "TFLOP/s", "kernel": "getrf_blocked_kernel<1,16> (FP64 VALU fma; FP64 peak)",
+            "unit": "TFLOP/s", "kernel": "getrf_mfma_kernel<256> (FP64 MFMA trailing update; FP64 peak)",
             "kernel_ms": fact_ms, "flops_per_launch": flops}
     value, ms = res["plain"]["solves_per_s"], plain_ms
     check = {}
@@ -355,11 +364,132 @@ def workload_vbatch(args, torch, ma, L, q):
                 check={}, step_ms=None, cpu=None)
 
 
+def serve_trace(seed, rate, seconds, pool, nlo=16, nhi=512, reuse=0.2):
+    """Poisson arrivals at `rate`/s for `seconds` s: per request (kind, n, nrhs,
+    matrix id, factor key, reuses).  New requests draw a matrix from the pool
+    and keep their factorization resident under their own key; a repeat
+    request (probability `reuse`) takes a recent request's matrix and key with
+    new right-hand sides."""
+    from magma_amd import gen
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, int(rate * seconds * 1.1) + 16))
+    arrivals = arrivals[arrivals < seconds]
+    m = len(arrivals)
+    pool_n = gen.vbatch_sizes(seed, pool, nlo, nhi)
+    pool_kind = (gen.scalar_u(seed + 1, np.arange(pool)) < 0.5).astype(np.int32)     # 1 = SPD
+    mat = rng.integers(0, pool, m)
+    key = np.arange(m, dtype=np.int64)
+    rep = rng.uniform(size=m) < reuse
+    back = rng.integers(1, 2000, m)
+    for i in np.nonzero(rep)[0]:
+        j = i - back[i]
+        if j >= 0:
+            mat[i] = mat[j]
+            key[i] = key[j]
+        else:
+            rep[i] = False
+    nrhs = rng.integers(1, 17, m).astype(np.int32)
+    return arrivals, mat, key, rep, nrhs, pool_n.astype(np.int32), pool_kind
+
+
+def workload_serve(args, torch, ma, L, q):
+    """Serving loop: Poisson arrivals, LU and SPD requests of varying n and
+    nrhs, FP64 answers by mixed-precision refinement, 20% repeat requests,
+    continuous batching under a 64 GB device budget (magma_amd_server_*)."""
+    import ctypes
+    from magma_amd import gen, lib
+    rate, seconds = 20000.0, args.serve_seconds
+    pool, ring = 4096, 32768
+    arrivals, mat, key, rep, nrhs, pool_n, pool_kind = serve_trace(4, rate, seconds, pool)
+    m = len(arrivals)
+    log(f"serve trace: {m} requests over {seconds} s, building the matrix pool")
+    # matrix pool on the device (column-major, ld = n)
+    offs = np.concatenate([[0], np.cumsum(pool_n.astype(np.int64) ** 2)[:-1]])
+    Apool = torch.empty(int((pool_n.astype(np.int64) ** 2).sum()), dtype=torch.float64, device="cuda")
+    for k0 in range(0, pool, 256):
+        for k in range(k0, min(pool, k0 + 256)):
+            n = int(pool_n[k])
+            if pool_kind[k]:
+                G = gen.normal_torch(4, k, 1, n * n, "cuda").view(n, n)
+                M = G @ G.T + n * torch.eye(n, dtype=torch.float64, device="cuda")
+            else:
+                M = gen.uniform_pm1_torch(4, k, 1, n * n, "cuda").view(n, n)
+            Apool[int(offs[k]): int(offs[k]) + n * n] = M.reshape(-1)
+    Bpool = gen.uniform_pm1_torch(104, 0, 1, 512 * 16 * 4096, "cuda").view(4096, 16, 512)
+    Xring = torch.zeros((ring, 16, 512), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    log("pool ready")
+    base_a, base_b, base_x = Apool.data_ptr(), Bpool.data_ptr(), Xring.data_ptr()
+    idx = np.arange(m)
+    rq = np.zeros(m, dtype=np.dtype([("kind", "i4"), ("n", "i4"), ("nrhs", "i4"), ("p0", "i4"),
+                                      ("dA", "u8"), ("ldda", "i4"), ("p1", "i4"), ("dB", "u8"), ("lddb", "i4"),
+                                      ("p2", "i4"), ("dX", "u8"), ("lddx", "i4"), ("p3", "i4"),
+                                      ("key", "i8")], align=True))
+    assert rq.dtype.itemsize == ctypes.sizeof(lib.Request)
+    n_req = pool_n[mat]
+    rq["kind"] = pool_kind[mat]
+    rq["n"] = n_req
+    rq["nrhs"] = nrhs
+    rq["dA"] = base_a + offs[mat].astype(np.uint64) * 8
+    rq["ldda"] = n_req
+    rq["dB"] = base_b + (idx % 4096).astype(np.uint64) * (16 * 512 * 8)
+    rq["lddb"] = 512
+    rq["dX"] = base_x + (idx % ring).astype(np.uint64) * (16 * 512 * 8)
+    rq["lddx"] = 512
+    rq["key"] = key
+    out = (lib.Completion * m)()
+    h = ctypes.c_void_p()
+    budget = 64 << 30
+    assert L.magma_amd_server_create(0, budget, 512, ctypes.byref(h)) == 0
+    try:
+        # warm-up: a short trace to create the streams' first kernels
+        warm = min(m, 2000)
+        warm_out = (lib.Completion * warm)()
+        wrq = rq[:warm].copy()
+        wrq["key"] = -1                    # nothing kept resident from the warm-up
+        warr = np.ascontiguousarray(arrivals[:warm] - arrivals[0])
+        assert L.magma_amd_server_run_trace(h, wrq.ctypes.data, warr.ctypes.data, warm, warm_out) == 0
+        log(f"warm-up done; replaying {m} requests")
+        t0 = time.perf_counter()
+        rc = L.magma_amd_server_run_trace(h, rq.ctypes.data, np.ascontiguousarray(arrivals).ctypes.data, m, out)
+        wall = time.perf_counter() - t0
+        assert rc == 0, rc
+        st = lib.ServerStats()
+        L.magma_amd_server_stats(h, ctypes.byref(st))
+    finally:
+        L.magma_amd_server_destroy(h)
+    res = np.frombuffer(out, dtype=np.dtype([("ticket", "i8"), ("info", "i4"), ("iter", "i4"), ("reused", "i4"),
+                                             ("pad", "i4"), ("berr", "f8"), ("ts", "f8"), ("tst", "f8"),
+                                             ("td", "f8")], align=True), count=m)
+    lat = (res["td"] - res["ts"]) * 1e3
+    span = float(res["td"].max() - res["ts"].min())
+    be_ratio = res["berr"] / (n_req * EPS)
+    conv = res["iter"] >= 0
+    log(f"served {m} in {wall:.1f} s: p50 {np.percentile(lat, 50):.3f} ms p99 {np.percentile(lat, 99):.3f} ms")
+    return dict(value=m / span, unit="solves/s", ms=span * 1e3 / m, dtype="f64 (mixed f32/f16 factor)",
+                config={"workload": "serve_poisson", "arrival_rate_per_s": rate, "seconds": seconds,
+                        "requests": m, "n": "log-uniform [16,512]", "nrhs": "uniform [1,16]",
+                        "spd_fraction": float(np.mean(pool_kind[mat])), "repeat_fraction": float(np.mean(rep)),
+                        "budget_bytes": budget, "latency_p50_ms": float(np.percentile(lat, 50)),
+                        "latency_p99_ms": float(np.percentile(lat, 99)), "latency_max_ms": float(lat.max()),
+                        "reused_factorizations": int(res["reused"].sum()),
+                        "fallbacks_fp64": int((~conv).sum()), "info_nonzero": int((res["info"] != 0).sum()),
+                        "backward_err_over_n_eps_max": float(be_ratio[conv].max()) if conv.any() else None,
+                        "backward_err_over_n_eps_p99": float(np.percentile(be_ratio[conv], 99)) if conv.any() else None,
+                        "server": {"batches": int(st.batches), "max_batch": int(st.max_batch_seen),
+                                   "evictions": int(st.evictions), "bytes_peak": int(st.bytes_peak),
+                                   "resident_at_end": int(st.resident)}},
+                roofline={"bound": "latency", "achieved": m / span, "peak": rate, "unit": "requests/s",
+                          "kernel": "mixed_kernel (per request workgroup)", "note": "offered load bound"},
+                check={}, step_ms=None, cpu=None)
+
+
 WORKLOADS = {
     "cond256_mixed_vs_plain": workload_cond256,
     "lu32_gesv": workload_lu32,
     "chol64_posv": workload_chol64,
     "vbatch_lu_gemm": workload_vbatch,
+    "serve_poisson": workload_serve,
 }
 
 
@@ -376,15 +506,28 @@ def cpu_baseline(spec, seconds):
     k = max(threads, 8)
     A, B = spec["gen"](k)
     t0 = time.perf_counter(); fn(A, B, nthreads=threads); dt = time.perf_counter() - t0
+    # at most 20000 distinct matrices in host memory; the sample is repeated
+    # until it holds about `seconds` of CPU work
     k2 = int(min(max(k, k * seconds / max(dt, 1e-6)), 20000))
     A, B = spec["gen"](k2)
     t0 = time.perf_counter(); fn(A, B, nthreads=threads); dt = time.perf_counter() - t0
+    reps = max(1, int(np.ceil(seconds / max(dt, 1e-6))) - 1)
+    log(f"cpu baseline: {k2} matrices x {reps + 1} passes")
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn(A, B, nthreads=threads)
+    dt = (dt + time.perf_counter() - t0) / (reps + 1)
     out = {"value": k2 / dt, "unit": "solves/s", "cores": threads, "kind": "port",
-           "sample": f"{k2} matrices of the workload (n={n}, nrhs={nrhs}), oracle {kind} "
-                     f"(netlib-order C restatement) under OpenMP, {threads} threads, {dt:.1f} s"}
+           "sample": f"{k2} matrices of the workload (n={n}, nrhs={nrhs}) x {reps + 1} passes, oracle {kind} "
+                     f"(netlib-order C restatement) under OpenMP, {threads} threads, "
+                     f"{dt * (reps + 1):.1f} s"}
     cfg = pyoracle.lapack()
     if cfg:
-        t0 = time.perf_counter(); lfn(A, B, nthreads=threads); dl = time.perf_counter() - t0
+        lreps = max(1, reps + 1)
+        t0 = time.perf_counter()
+        for _ in range(lreps):
+            lfn(A, B, nthreads=threads)
+        dl = (time.perf_counter() - t0) / lreps
         out["lapack"] = {"value": k2 / dl, "unit": "solves/s", "library": cfg.split()[0] if cfg else "",
                          "config": cfg, "cores": threads}
     return out
@@ -398,6 +541,7 @@ def main():
     ap.add_argument("--workload", default="cond256_mixed_vs_plain", choices=sorted(WORKLOADS))
     ap.add_argument("--batch", type=int, default=0, help="override the batch count (cond256 only)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-seconds", type=float, default=30.0, help="trace length of serve_poisson")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", dest="check", action="store_false")
     args = ap.parse_args()
@@ -412,6 +556,7 @@ def main():
     q = ma.Queue(0)
     L.magma_queue_set_async(q.handle, 1)      # back-to-back submission; synchronize around the timed region
 
+    log(f"workload {args.workload}")
     r = WORKLOADS[args.workload](args, torch, ma, L, q)
     roof = r["roofline"]
     roof["frac"] = roof["achieved"] / roof["peak"]

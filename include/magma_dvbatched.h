@@ -74,6 +74,16 @@ magma_int_t magma_dgetrs_vbatched(magma_trans_t trans, magma_int_t *n, magma_int
 magma_int_t magma_dpotrs_vbatched(magma_uplo_t uplo, magma_int_t *n, magma_int_t *nrhs,
                                   double **dA_array, magma_int_t *ldda, double **dB_array, magma_int_t *lddb,
                                   magma_int_t batchCount, magma_queue_t queue);
+/* New: the two solves without argument checks and without any host
+ * synchronization (graph-capturable); max_n bounds every n[i]. */
+magma_int_t magma_dgetrs_vbatched_max_nocheck(magma_trans_t trans, magma_int_t *n, magma_int_t *nrhs,
+                                              double **dA_array, magma_int_t *ldda, magma_int_t **dipiv_array,
+                                              double **dB_array, magma_int_t *lddb,
+                                              magma_int_t batchCount, magma_int_t max_n, magma_queue_t queue);
+magma_int_t magma_dpotrs_vbatched_max_nocheck(magma_uplo_t uplo, magma_int_t *n, magma_int_t *nrhs,
+                                              double **dA_array, magma_int_t *ldda, double **dB_array,
+                                              magma_int_t *lddb, magma_int_t batchCount, magma_int_t max_n,
+                                              magma_queue_t queue);
 
 #ifdef __cplusplus
 }

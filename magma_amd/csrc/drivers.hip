@@ -5,7 +5,8 @@
 // Dispatch (replaces the static crossover tables of
 // control/get_batched_crossover.cpp:25-228 and control/batched_kernel_param.h):
 //   max(m,n) <= 64   register-resident kernels, 64/NMAX matrices per wavefront
-//   otherwise        blocked workgroup-per-matrix kernels
+//   max(m,n) <= 512  LU: workgroup per matrix, FP64 MFMA trailing update
+//   otherwise        blocked workgroup-per-matrix kernels (VALU)
 #include <algorithm>
 
 #include "kernels.h"
@@ -15,9 +16,11 @@ namespace magma_amd {
 magma_int_t launch_getrf_small(const MatSet &s, int max_mn_dim, bool solve, hipStream_t st);
 magma_int_t launch_potrf_small(const MatSet &s, int max_n, bool solve, hipStream_t st);
 magma_int_t launch_getrf_blocked(const MatSet &s, int max_m, hipStream_t st);
+magma_int_t launch_getrf_mfma(const MatSet &s, int max_dim, hipStream_t st);
 magma_int_t launch_potrf_blocked(const MatSet &s, int max_n, hipStream_t st);
 
 constexpr int kRegMax = 64;
+constexpr int kMfmaMax = 512;
 
 namespace {
 __global__ void fill_int_kernel(int *p, int v, int count)
@@ -40,7 +43,7 @@ magma_int_t launch_getrf(const MatSet &s, int max_m, int max_n, bool fused_solve
 {
     const int dim = std::max(max_m, max_n);
     if (dim <= kRegMax) return launch_getrf_small(s, dim, fused_solve, st);
-    magma_int_t r = launch_getrf_blocked(s, max_m, st);
+    magma_int_t r = (dim <= kMfmaMax) ? launch_getrf_mfma(s, dim, st) : launch_getrf_blocked(s, max_m, st);
     if (r != MAGMA_SUCCESS || !fused_solve) return r;
     return launch_getrs(s, 'N', max_n, 0, st);
 }

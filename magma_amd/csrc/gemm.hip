@@ -23,13 +23,13 @@ namespace {
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int kWaves = 4;
-constexpr int kTile = 32;
 
 __device__ __forceinline__ int pick(const int *arr, int fixed, int id) { return arr ? arr[id] : fixed; }
 
-template <bool TA, bool TB>
+template <bool TA, bool TB, int BT>
 __global__ void __launch_bounds__(64 * kWaves) gemm_mfma_kernel(GemmSet g, const int *prefix, int fixed_tiles)
 {
+    constexpr int TS = 16 * BT;        // tile edge: BT x BT MFMA blocks of 16x16
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int li = lane & 15;          // row within a 16-block (A) / column (B, D)
@@ -55,41 +55,44 @@ __global__ void __launch_bounds__(64 * kWaves) gemm_mfma_kernel(GemmSet g, const
         const int m = pick(g.m_arr, g.m, id), n = pick(g.n_arr, g.n, id), k = pick(g.k_arr, g.k, id);
         const int lda = pick(g.lda_arr, g.lda, id), ldb = pick(g.ldb_arr, g.ldb, id);
         const int ldc = pick(g.ldc_arr, g.ldc, id);
-        const int mt = (m + kTile - 1) / kTile;
-        const int row0 = (local % mt) * kTile;
-        const int col0 = (local / mt) * kTile;
+        const int mt = (m + TS - 1) / TS;
+        const int row0 = (local % mt) * TS;
+        const int col0 = (local / mt) * TS;
         const double *A = g.A[id];
         const double *B = g.B[id];
         double *C = g.C[id];
 
-        d4 acc[2][2];
+        d4 acc[BT][BT];
 #pragma unroll
-        for (int bi = 0; bi < 2; ++bi)
+        for (int bi = 0; bi < BT; ++bi)
 #pragma unroll
-            for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = d4{0.0, 0.0, 0.0, 0.0};
+            for (int bj = 0; bj < BT; ++bj) acc[bi][bj] = d4{0.0, 0.0, 0.0, 0.0};
 
-        const int r0 = row0 + li, r1 = row0 + 16 + li;
-        const int c0 = col0 + li, c1 = col0 + 16 + li;
         for (int k0 = 0; k0 < k; k0 += 4) {
             const int kk = k0 + lk;
             const bool kin = kk < k;
-            double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
-            if (kin) {
-                if (r0 < m) a0 = TA ? A[kk + (size_t)r0 * lda] : A[r0 + (size_t)kk * lda];
-                if (r1 < m) a1 = TA ? A[kk + (size_t)r1 * lda] : A[r1 + (size_t)kk * lda];
-                if (c0 < n) b0 = TB ? B[c0 + (size_t)kk * ldb] : B[kk + (size_t)c0 * ldb];
-                if (c1 < n) b1 = TB ? B[c1 + (size_t)kk * ldb] : B[kk + (size_t)c1 * ldb];
+            double a[BT], b[BT];
+#pragma unroll
+            for (int q = 0; q < BT; ++q) {
+                const int r = row0 + 16 * q + li, c = col0 + 16 * q + li;
+                a[q] = 0.0;
+                b[q] = 0.0;
+                if (kin) {
+                    if (r < m) a[q] = TA ? A[kk + (size_t)r * lda] : A[r + (size_t)kk * lda];
+                    if (c < n) b[q] = TB ? B[c + (size_t)kk * ldb] : B[kk + (size_t)c * ldb];
+                }
             }
-            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+#pragma unroll
+            for (int bi = 0; bi < BT; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < BT; ++bj)
+                    acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[bi], b[bj], acc[bi][bj], 0, 0, 0);
         }
         const double alpha = g.alpha, beta = g.beta;
 #pragma unroll
-        for (int bi = 0; bi < 2; ++bi) {
+        for (int bi = 0; bi < BT; ++bi) {
 #pragma unroll
-            for (int bj = 0; bj < 2; ++bj) {
+            for (int bj = 0; bj < BT; ++bj) {
                 const int col = col0 + 16 * bj + li;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
@@ -106,7 +109,7 @@ __global__ void __launch_bounds__(64 * kWaves) gemm_mfma_kernel(GemmSet g, const
 }
 
 // C = beta C when k == 0 or alpha == 0 (BLAS quick path)
-__global__ void gemm_scale_kernel(GemmSet g, const int *prefix, int fixed_tiles)
+__global__ void gemm_scale_kernel(GemmSet g, const int *prefix, int fixed_tiles, int ts)
 {
     const int total = prefix ? prefix[g.count] : fixed_tiles * g.count;
     for (int t = blockIdx.x; t < total; t += gridDim.x) {
@@ -126,11 +129,11 @@ __global__ void gemm_scale_kernel(GemmSet g, const int *prefix, int fixed_tiles)
         const int id = g.index ? g.index[s] : s;
         const int m = pick(g.m_arr, g.m, id), n = pick(g.n_arr, g.n, id);
         const int ldc = pick(g.ldc_arr, g.ldc, id);
-        const int mt = (m + kTile - 1) / kTile;
-        const int row0 = (local % mt) * kTile, col0 = (local / mt) * kTile;
+        const int mt = (m + ts - 1) / ts;
+        const int row0 = (local % mt) * ts, col0 = (local / mt) * ts;
         double *C = g.C[id];
-        for (int e = threadIdx.x; e < kTile * kTile; e += blockDim.x) {
-            const int row = row0 + e % kTile, col = col0 + e / kTile;
+        for (int e = threadIdx.x; e < ts * ts; e += blockDim.x) {
+            const int row = row0 + e % ts, col = col0 + e / ts;
             if (row < m && col < n) {
                 double *c = C + row + (size_t)col * ldc;
                 *c = (g.beta == 0.0) ? 0.0 : g.beta * *c;
@@ -141,26 +144,32 @@ __global__ void gemm_scale_kernel(GemmSet g, const int *prefix, int fixed_tiles)
 
 }  // namespace
 
-// prefix: device tile prefix (vbatched) or null (fixed sizes).  max_tiles
-// bounds the flat tile count for the grid size.
-magma_int_t launch_gemm_tiles(const GemmSet &g, const int *prefix, long max_tiles, hipStream_t st)
+template <int BT>
+static void launch_bt(const GemmSet &g, const int *prefix, int fixed_tiles, int grid, hipStream_t st)
+{
+    if (g.ta == 'N' && g.tb == 'N') gemm_mfma_kernel<false, false, BT><<<grid, 64 * kWaves, 0, st>>>(g, prefix, fixed_tiles);
+    else if (g.ta == 'N') gemm_mfma_kernel<false, true, BT><<<grid, 64 * kWaves, 0, st>>>(g, prefix, fixed_tiles);
+    else if (g.tb == 'N') gemm_mfma_kernel<true, false, BT><<<grid, 64 * kWaves, 0, st>>>(g, prefix, fixed_tiles);
+    else gemm_mfma_kernel<true, true, BT><<<grid, 64 * kWaves, 0, st>>>(g, prefix, fixed_tiles);
+}
+
+// prefix: device tile prefix (vbatched) or null (fixed sizes), built for tile
+// edge `tile` (16 or 32).  max_tiles bounds the flat tile count for the grid size.
+magma_int_t launch_gemm_tiles(const GemmSet &g, const int *prefix, long max_tiles, int tile, hipStream_t st)
 {
     if (g.count <= 0 || max_tiles <= 0) return MAGMA_SUCCESS;
-    const int fixed_tiles = prefix ? 0 : ceil_div(g.m, kTile) * ceil_div(g.n, kTile);
+    if (tile != 16 && tile != 32) return MAGMA_ERR_NOT_SUPPORTED;
+    const int fixed_tiles = prefix ? 0 : ceil_div(g.m, tile) * ceil_div(g.n, tile);
     // enough resident wavefronts to fill 256 CUs several times over
     const long want = (max_tiles + kWaves - 1) / kWaves;
     const int grid = (int)std::min<long>(want, 256L * 16);
     const bool scale_only = (!prefix && (g.k == 0 || g.alpha == 0.0)) || g.alpha == 0.0;
     if (scale_only) {
-        gemm_scale_kernel<<<grid, 256, 0, st>>>(g, prefix, fixed_tiles);
-    } else if (g.ta == 'N' && g.tb == 'N') {
-        gemm_mfma_kernel<false, false><<<grid, 64 * kWaves, 0, st>>>(g, prefix, fixed_tiles);
-    } else if (g.ta == 'N') {
-        gemm_mfma_kernel<false, true><<<grid, 64 * kWaves, 0, st>>>(g, prefix, fixed_tiles);
-    } else if (g.tb == 'N') {
-        gemm_mfma_kernel<true, false><<<grid, 64 * kWaves, 0, st>>>(g, prefix, fixed_tiles);
+        gemm_scale_kernel<<<grid, 256, 0, st>>>(g, prefix, fixed_tiles, tile);
+    } else if (tile == 32) {
+        launch_bt<2>(g, prefix, fixed_tiles, grid, st);
     } else {
-        gemm_mfma_kernel<true, true><<<grid, 64 * kWaves, 0, st>>>(g, prefix, fixed_tiles);
+        launch_bt<1>(g, prefix, fixed_tiles, grid, st);
     }
     MAGMA_LAUNCH_CHECK();
     return MAGMA_SUCCESS;

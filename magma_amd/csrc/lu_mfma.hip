@@ -1,0 +1,395 @@
+// lu_mfma.hip -- blocked right-looking LU with partial pivoting, FP64, one
+// 256-thread workgroup (4 wavefronts) per matrix, for 64 < max(m, n) <= 512.
+//
+// Per panel of PW = 32 columns (rows j0..m-1), three phases separated by
+// workgroup barriers:
+//
+//   panel  The panel lives in registers, row pr = tid + 256 i of it in thread
+//          tid.  Per column: local (|a|, position) key, wavefront arg-max by
+//          DPP (quad, half-row and row mirrors) plus a read of the four row
+//          leaders, then each wavefront's winning row is published in an LDS
+//          slot (double-buffered by column parity) and ONE barrier later every
+//          thread picks the winning wavefront's row as the pivot row.  Row
+//          interchanges inside the panel are logical (pos); rows are written
+//          back to their final positions once, at the end of the panel.
+//   rows   One thread per column outside the panel: the panel's interchanges
+//          as a gather (all loads, then all stores, so nothing is read after
+//          it is overwritten), and for the columns right of the panel the
+//          unit-lower solve U12 = L11^-1 A12 in registers, L11 from LDS.
+//          -U12 is also staged in LDS for the next phase.
+//   update A22 -= L21 U12 in 32x32 tiles, one tile per wavefront at a time, on
+//          v_mfma_f64_16x16x4f64.  The product is formed transposed
+//          (A22^T -= U12^T L21^T) so the C tile and the L21 operand are read
+//          as contiguous 128-byte column segments; -U12^T comes from LDS.
+//
+// Replaces the reference's launch chain for these sizes (panel + laswp + trsm
+// + gemm, src/zgetrf_batched.cpp:222-351, src/zgetrf_panel_batched.cpp:127-243,
+// magmablas/zgetf2_kernels.cu:353-760) with one persistent launch.
+//
+// Rounding: every entry receives its updates as single fused multiply-adds
+// in increasing k, in the panel (VALU), the solve (VALU) and the update: the
+// FP64 MFMA applies its four k terms of a slice as the sequential fma chain,
+// slices in increasing k.  Multipliers are a * (1/pivot) (division below
+// DBL_MIN), the pivot is the first maximum of |a| with NaN taken only in the
+// first position.  So factors and pivots are the bits of netlib dgetrf with
+// that update order.  When a multiplier of the panel is not finite the
+// update and solve switch to VALU loops that skip zero U entries as the
+// reference BLAS loops do (only then can the skip change a result).
+#include <cfloat>
+#include <climits>
+
+#include "kernels.h"
+#include "tune.h"
+
+namespace magma_amd {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / 64;
+constexpr int PW = 32;
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ int pick(const int *arr, int fixed, int id) { return arr ? arr[id] : fixed; }
+
+// (key, pos) order of the pivot search: larger key wins, ties to the smaller
+// logical position (first maximum).  Keys are never NaN.
+__device__ __forceinline__ void better(double &key, int &pos, double k2, int p2)
+{
+    const bool take = (k2 > key) || (k2 == key && p2 < pos);
+    key = take ? k2 : key;
+    pos = take ? p2 : pos;
+}
+
+template <int CTRL>
+__device__ __forceinline__ void argmax_dpp(double &key, int &pos)
+{
+    const unsigned long long kb = (unsigned long long)__double_as_longlong(key);
+    const int lo = __builtin_amdgcn_update_dpp((int)kb, (int)kb, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(kb >> 32), (int)(kb >> 32), CTRL, 0xF, 0xF, false);
+    const int p = __builtin_amdgcn_update_dpp(pos, pos, CTRL, 0xF, 0xF, false);
+    const double k2 = __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+    better(key, pos, k2, p);
+}
+
+__device__ __forceinline__ double readlane_d(double v, int lane)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = __builtin_amdgcn_readlane((int)b, lane);
+    const unsigned hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Wavefront arg-max of (key, pos); the result is wave-uniform.
+__device__ __forceinline__ void wave_argmax(double &key, int &pos)
+{
+    argmax_dpp<0xB1>(key, pos);    // quad_perm [1,0,3,2]
+    argmax_dpp<0x4E>(key, pos);    // quad_perm [2,3,0,1]
+    argmax_dpp<0x141>(key, pos);   // row_half_mirror
+    argmax_dpp<0x140>(key, pos);   // row_mirror: every lane holds its row's best
+    double k = readlane_d(key, 0);
+    int p = __builtin_amdgcn_readlane(pos, 0);
+#pragma unroll
+    for (int r = 1; r < 4; ++r) better(k, p, readlane_d(key, 16 * r), __builtin_amdgcn_readlane(pos, 16 * r));
+    key = k;
+    pos = p;
+}
+
+struct PivSlot {
+    double key;
+    int pos;
+    int pad;
+    double row[PW];
+};
+
+template <int MT>
+struct LuSmem {
+    PivSlot slot[2][kWaves];
+    double L11[PW][PW + 1];
+    double Us[PW][MT];        // -U12 of the current panel, [k][column - (j0 + jb)]
+    int src_top[PW];          // logical panel row r came from physical panel row src_top[r]
+    int down_dst[PW];         // physical panel row r (< jb) moves to logical row down_dst[r] (>= jb), or -1
+    int swaps[PW];
+};
+
+template <int MT>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) getrf_mfma_kernel(MatSet s)
+{
+    constexpr int RPT = MT / kThreads;
+    __shared__ LuSmem<MT> sm;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int w = tid >> 6;
+    const int li = lane & 15;
+    const int lk = lane >> 4;
+    const int cnt = item_count(s);
+    const int *idx = item_index(s);
+    for (int item = blockIdx.x; item < cnt; item += gridDim.x) {
+        const int id = idx ? idx[item] : item;
+        const int m = pick(s.m_arr, s.m, id);
+        const int n = pick(s.n_arr, s.n, id);
+        const int lda = pick(s.lda_arr, s.lda, id);
+        double *A = s.A[id];
+        int *ipiv = s.ipiv[id];
+        const int mn = min(m, n);
+        int info = 0;
+
+        for (int j0 = 0; j0 < mn; j0 += PW) {
+            const int jb = min(PW, mn - j0);
+            const int mp = m - j0;
+            // ------------------------------------------------ panel --------
+            double a[RPT][PW];
+            int pos[RPT];
+#pragma unroll
+            for (int i = 0; i < RPT; ++i) {
+                const int pr = tid + kThreads * i;
+                pos[i] = pr;
+#pragma unroll
+                for (int c = 0; c < PW; ++c)
+                    a[i][c] = (pr < mp && c < jb) ? A[(j0 + pr) + (size_t)(j0 + c) * lda] : 0.0;
+            }
+            bool bad = false;
+#pragma unroll
+            for (int jj = 0; jj < PW; ++jj) {
+                if (jj < jb) {
+                    double key = -1.0;
+                    int kp = INT_MAX;
+                    int own = 0;
+#pragma unroll
+                    for (int i = 0; i < RPT; ++i) {
+                        const int pr = tid + kThreads * i;
+                        const bool cand = pr < mp && pos[i] >= jj;
+                        const double v = a[i][jj];
+                        double k2 = -1.0;
+                        if (cand) k2 = (v != v) ? ((pos[i] == jj) ? __builtin_huge_val() : -1.0) : fabs(v);
+                        const double kold = key;
+                        const int pold = kp;
+                        better(key, kp, k2, cand ? pos[i] : INT_MAX);
+                        if (key != kold || kp != pold) own = i;
+                    }
+                    double wk = key;
+                    int wp = kp;
+                    wave_argmax(wk, wp);
+                    PivSlot &sl = sm.slot[jj & 1][w];
+                    if (wp != INT_MAX && kp == wp) {
+#pragma unroll
+                        for (int c = jj; c < PW; ++c) {
+                            double v = a[0][c];
+#pragma unroll
+                            for (int i = 1; i < RPT; ++i) v = (own == i) ? a[i][c] : v;
+                            sl.row[c] = v;
+                        }
+                    }
+                    if (lane == 0) {
+                        sl.key = wk;
+                        sl.pos = wp;
+                    }
+                    __syncthreads();
+                    double bk = sm.slot[jj & 1][0].key;
+                    int bp = sm.slot[jj & 1][0].pos;
+                    int bw = 0;
+#pragma unroll
+                    for (int q = 1; q < kWaves; ++q) {
+                        const double k2 = sm.slot[jj & 1][q].key;
+                        const int p2 = sm.slot[jj & 1][q].pos;
+                        const bool take = (k2 > bk) || (k2 == bk && p2 < bp);
+                        bk = take ? k2 : bk;
+                        bp = take ? p2 : bp;
+                        bw = take ? q : bw;
+                    }
+                    const double *prow = sm.slot[jj & 1][bw].row;
+                    const int p = bp;
+                    if (tid == 0) sm.swaps[jj] = p;
+                    const double piv = prow[jj];
+                    if (piv == 0.0 && info == 0) info = j0 + jj + 1;
+                    const bool use_rcp = fabs(piv) >= DBL_MIN;
+                    const double rc = 1.0 / piv;
+#pragma unroll
+                    for (int i = 0; i < RPT; ++i) {
+                        const int pr = tid + kThreads * i;
+                        pos[i] = (pos[i] == jj) ? p : ((pos[i] == p) ? jj : pos[i]);
+                        const bool below = pr < mp && pos[i] > jj;
+                        if (below) {
+                            double l = a[i][jj];
+                            if (piv != 0.0) l = use_rcp ? rc * l : l / piv;
+                            a[i][jj] = l;
+                            bad |= !__builtin_isfinite(l);
+#pragma unroll
+                            for (int c = jj + 1; c < PW; ++c) {
+                                const double u = prow[c];
+                                if (u != 0.0) a[i][c] = fma(l, -u, a[i][c]);
+                            }
+                        }
+                    }
+                }
+            }
+            // panel rows to their final positions; L11, the interchange maps
+#pragma unroll
+            for (int i = 0; i < RPT; ++i) {
+                const int pr = tid + kThreads * i;
+                if (pr < mp) {
+#pragma unroll
+                    for (int c = 0; c < PW; ++c)
+                        if (c < jb) A[(j0 + pos[i]) + (size_t)(j0 + c) * lda] = a[i][c];
+                    if (pos[i] < jb) {
+#pragma unroll
+                        for (int c = 0; c < PW; ++c) sm.L11[pos[i]][c] = a[i][c];
+                        sm.src_top[pos[i]] = pr;
+                    }
+                    if (pr < jb) sm.down_dst[pr] = (pos[i] >= jb) ? pos[i] : -1;
+                }
+            }
+            if (tid < jb) ipiv[j0 + tid] = j0 + sm.swaps[tid] + 1;
+            const bool exact = __syncthreads_or(bad);
+
+            // ------------------------------------------------ rows ---------
+            const int cb = j0 + jb;                 // first column right of the panel
+            const int nother = n - jb;
+            for (int t = tid; t < nother; t += kThreads) {
+                const int c = (t < j0) ? t : t + jb;
+                double *col = A + (size_t)c * lda + j0;
+                double x[PW], wv[PW];
+#pragma unroll
+                for (int r = 0; r < PW; ++r) x[r] = (r < jb) ? col[sm.src_top[r]] : 0.0;
+#pragma unroll
+                for (int r = 0; r < PW; ++r) wv[r] = (r < jb && sm.down_dst[r] >= 0) ? col[r] : 0.0;
+#pragma unroll
+                for (int r = 0; r < PW; ++r)
+                    if (r < jb && sm.down_dst[r] >= 0) col[sm.down_dst[r]] = wv[r];
+                if (c >= cb) {
+                    // rows of L11 at and beyond jb are never read for k < jb
+                    // into a stored result: x[k] = 0 there
+                    if (!exact) {
+#pragma unroll
+                        for (int k = 0; k < PW; ++k) {
+                            const double u = -x[k];
+#pragma unroll
+                            for (int i = k + 1; i < PW; ++i) x[i] = fma(sm.L11[i][k], u, x[i]);
+                            __builtin_amdgcn_sched_barrier(0);
+                        }
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < PW; ++k) {
+                            const double u = x[k];
+#pragma unroll
+                            for (int i = k + 1; i < PW; ++i)
+                                if (u != 0.0) x[i] = fma(sm.L11[i][k], -u, x[i]);
+                            __builtin_amdgcn_sched_barrier(0);
+                        }
+                    }
+#pragma unroll
+                    for (int k = 0; k < PW; ++k) sm.Us[k][c - cb] = (k < jb) ? -x[k] : -0.0;
+                }
+#pragma unroll
+                for (int r = 0; r < PW; ++r)
+                    if (r < jb) col[r] = x[r];
+            }
+            __syncthreads();
+
+            // ------------------------------------------------ update -------
+            const int nrt = (m - cb + 31) / 32;
+            const int nct = (n - cb + 31) / 32;
+            if (m > cb && n > cb) {
+                for (int t = w; t < nrt * nct; t += kWaves) {
+                    const int r0 = cb + 32 * (t % nrt);
+                    const int c0 = cb + 32 * (t / nrt);
+                    if (!exact) {
+                        // D = C^T: lane (lk, li), register i of block (bi, bj)
+                        // holds C(r0 + 16 bj + li, c0 + 16 bi + lk + 4 i)
+                        d4 acc[2][2];
+#pragma unroll
+                        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                            for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                                for (int i = 0; i < 4; ++i) {
+                                    const int row = r0 + 16 * bj + li;
+                                    const int col = c0 + 16 * bi + lk + 4 * i;
+                                    acc[bi][bj][i] = (row < m && col < n) ? A[row + (size_t)col * lda] : 0.0;
+                                }
+#pragma unroll
+                        for (int k0 = 0; k0 < PW; k0 += 4) {
+                            const int kk = k0 + lk;
+                            double ua[2], lb[2];
+#pragma unroll
+                            for (int bi = 0; bi < 2; ++bi) {
+                                const int col = c0 + 16 * bi + li;
+                                ua[bi] = (col < n) ? sm.Us[kk][col - cb] : -0.0;
+                            }
+#pragma unroll
+                            for (int bj = 0; bj < 2; ++bj) {
+                                const int row = r0 + 16 * bj + li;
+                                lb[bj] = (row < m && kk < jb) ? A[row + (size_t)(j0 + kk) * lda] : 0.0;
+                            }
+#pragma unroll
+                            for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                                for (int bj = 0; bj < 2; ++bj)
+                                    acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(ua[bi], lb[bj], acc[bi][bj], 0, 0, 0);
+                        }
+#pragma unroll
+                        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                            for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                                for (int i = 0; i < 4; ++i) {
+                                    const int row = r0 + 16 * bj + li;
+                                    const int col = c0 + 16 * bi + lk + 4 * i;
+                                    if (row < m && col < n) A[row + (size_t)col * lda] = acc[bi][bj][i];
+                                }
+                    } else {
+                        // reference BLAS loop semantics: skip a zero U entry
+                        for (int e = lane; e < 32 * 32; e += 64) {
+                            const int row = r0 + (e & 31), col = c0 + (e >> 5);
+                            if (row < m && col < n) {
+                                double v = A[row + (size_t)col * lda];
+                                for (int k = 0; k < jb; ++k) {
+                                    const double u = -sm.Us[k][col - cb];
+                                    if (u != 0.0) v = fma(A[row + (size_t)(j0 + k) * lda], -u, v);
+                                }
+                                A[row + (size_t)col * lda] = v;
+                            }
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        if (tid == 0) s.info[id] = info;
+    }
+}
+
+}  // namespace
+
+template <class K>
+static int resident_per_cu(K kernel)
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, kThreads, 0) != hipSuccess || nb < 1) nb = 1;
+    return nb;
+}
+
+magma_int_t launch_getrf_blocked(const MatSet &s, int max_m, hipStream_t st);
+
+// 64 < max(m, n) <= 512; returns MAGMA_ERR_NOT_SUPPORTED above.  Variant
+// (tuned per size class): 0 as many workgroups per CU as fit, 1 one workgroup
+// per CU, 2 the VALU blocked kernel.
+magma_int_t launch_getrf_mfma(const MatSet &s, int max_dim, hipStream_t st)
+{
+    if (s.count <= 0) return MAGMA_SUCCESS;
+    if (max_dim > 512) return MAGMA_ERR_NOT_SUPPORTED;
+    const int v = tuned_variant(kTuneGetrf, size_class_of(max_dim));
+    if (v == 2) return launch_getrf_blocked(s, max_dim, st);
+    if (max_dim <= 256) {
+        const int grid = persistent_grid(s.count, v == 1 ? 1 : resident_per_cu(getrf_mfma_kernel<256>));
+        getrf_mfma_kernel<256><<<grid, kThreads, 0, st>>>(s);
+    } else {
+        const int grid = persistent_grid(s.count, v == 1 ? 1 : resident_per_cu(getrf_mfma_kernel<512>));
+        getrf_mfma_kernel<512><<<grid, kThreads, 0, st>>>(s);
+    }
+    MAGMA_LAUNCH_CHECK();
+    return MAGMA_SUCCESS;
+}
+
+}  // namespace magma_amd

@@ -1,520 +1,1043 @@
-// mixed.hip -- batched mixed-precision solve (dsgesv semantics, batched).
+// mixed.hip -- batched and variable-size mixed-precision solves with the
+// semantics of dsgesv / dsposv (src/zcgesv_gpu.cpp:143-365, src/zcposv_gpu.cpp):
+// factor in FP32, refine in FP64, fall back to FP64 when refinement fails.
 //
-// Per matrix, one 256-thread workgroup (persistent over the batch):
-//   1. ||A||_inf, max|a_ij| (FP64); W = fp32(A * 2^-e), 2^e >= max|a_ij| a
-//      power of two, so every scaled entry is below 1 in magnitude and the FP16
-//      operands below cannot overflow.
-//   2. LU with partial pivoting of W in FP32, blocked by 32 columns: the panel
-//      is factored in registers (VALU FP32), the trailing update
-//      A22 -= L21 U12 runs on FP16 MFMA (v_mfma_f32_16x16x16f16) with each
-//      operand split hi + lo (hi = fp16(x), lo = fp16(x - hi)) and three
-//      products hi*hi + hi*lo + lo*hi accumulated in FP32, which keeps about 22
-//      bits of each operand: plain FP16 or BF16 operands would leave the
-//      refinement divergent for condition numbers above ~1e4.
-//   3. x = U^-1 L^-1 P b in FP32; then the FP64 refinement loop of dsgesv
-//      (src/zcgesv_gpu.cpp:143-365): r = b - A x in FP64, stop when
-//      ||r_j||_inf <= ||x_j||_inf ||A||_inf eps sqrt(n) for every column j,
-//      else d = LU32 \ r, x += d (FP64), at most MAGMA_MIXED_ITERMAX sweeps.
-//   4. Matrices that do not converge (or whose FP32 factorization meets a zero
-//      pivot) are appended to a device list; the plain FP64 getrf + getrs
-//      kernels then run on exactly that list (no host round trip).
+// One 256-thread workgroup per matrix (persistent over the work list), n <= 512.
+//
+// Factorization (left-looking by block columns of NB, 32-wide chunks):
+//   * rows stay in the input's original order for the whole factorization;
+//     a pivot is recorded (pivot order p, pivot position of each row) instead
+//     of interchanging rows, so no row is ever moved in memory.  LAPACK-form
+//     ipiv is produced on the fly from each row's current position, and the
+//     pivot search breaks ties by current position as idamax does.
+//   * a block column (all n rows x NB columns, FP32) lives in LDS.  It is read
+//     once from the FP64 input, each column scaled by a power of two so that
+//     its largest entry is in [0.5, 1) (Cholesky: the symmetric scaling
+//     d_i = 2^-e_i with a_ii 2^-2e_i in [0.5, 2)), then updated by every
+//     previous chunk of the factor (read back from the factor storage in
+//     HBM), then factored chunk by chunk, then written out once.
+//   * applying a chunk: the 32 pivot rows get the unit-lower triangular
+//     solve (LU), then every row still to be pivoted gets C -= L U on
+//     v_mfma_f32_16x16x16f16 with both operands split hi + lo in FP16 (three
+//     products, FP32 accumulation), computed transposed so that the rows of
+//     C are the fast MFMA index.  Rows already pivoted get a zero L operand.
+//   * the chunk panel (32 columns, all rows) is factored in registers, one
+//     workgroup barrier per column: wavefront arg-max by DPP, each
+//     wavefront publishes its best row in an LDS slot, and every thread picks
+//     the winning slot after the barrier.
+//
+// Refinement (all right-hand sides of a group at once, as dsgesv does):
+//   x = 0, r = b; repeat: d = (LU)^-1 r in FP32 (blocked sweeps over the pivot
+//   order: one wavefront per right-hand side solves each 32x32 diagonal block
+//   with lane broadcasts, the workgroup applies the block to the other rows),
+//   x += d in FP64, r = b - A x in FP64, stop when every column satisfies
+//   ||r||_inf <= ||x||_inf ||A||_inf eps sqrt(n) (src/zcgesv_gpu.cpp:266-279),
+//   at most MAGMA_MIXED_ITERMAX correction sweeps.
+//
+// Matrices whose FP32 factorization meets a zero pivot (-3), whose entries
+// cannot be represented (-2) or that do not converge (-(ITERMAX+1)) are
+// appended to a device list, and the plain FP64 kernels factor exactly those.
 #include <cfloat>
 #include <climits>
+#include <cstdio>
 
 #include <algorithm>
 
-#include "kernels.h"
+#include "magma_amd.h"
+#include "mixed.h"
+#include "tune.h"
 
 namespace magma_amd {
+
+magma_int_t launch_getrf_small_v(const MatSet &s, int max_mn_dim, bool solve, bool var, hipStream_t st);
+magma_int_t launch_potrf_small_v(const MatSet &s, int max_n, bool solve, bool var, hipStream_t st);
+magma_int_t launch_getrf_mfma(const MatSet &s, int max_dim, hipStream_t st);
+magma_int_t launch_getrf_blocked(const MatSet &s, int max_m, hipStream_t st);
+magma_int_t launch_potrf_blocked(const MatSet &s, int max_n, hipStream_t st);
 
 namespace {
 
 typedef _Float16 half4 __attribute__((ext_vector_type(4)));
 typedef float float4v __attribute__((ext_vector_type(4)));
 
-constexpr int kT = 256;        // threads per workgroup
-constexpr int kPW = 32;        // panel width
-constexpr int kTile = 64;      // trailing-update tile (rows and columns)
-constexpr int kMaxN = 512;
+constexpr int kT = 256;
+constexpr int kWaves = kT / 64;
+constexpr int CH = 32;          // chunk width
+constexpr int kUPad = CH + 4;   // staging row (halves): 72-byte stride, conflict-free b64 reads
 
-struct MixedArgs {
-    int n, nrhs, lda, ldb, ldx;
-    double **A, **B, **X;
-    int **ipiv;
-    int *info;
-    int *iter;
-    float *W;            // one n x n FP32 slot per workgroup
-    int *fb_list;        // matrices that fall back to FP64
-    int *fb_count;
-    int count;
+__device__ __forceinline__ int pick(const int *arr, int fixed, int id) { return arr ? arr[id] : fixed; }
+
+struct FSlot {
+    float key;
+    int cur;
+    int row;
+    int pad;
+    float val[CH];
 };
 
-// max over the workgroup that propagates NaN (fmax would drop it, and a NaN
-// residual must never pass the convergence test)
-__device__ __forceinline__ double nanmax(double a, double b) { return (a > b || a != a) ? a : b; }
-
-__device__ __forceinline__ double block_max(double v, double *red)
-{
-    for (int off = 32; off > 0; off >>= 1) v = nanmax(v, __shfl_xor(v, off));
-    const int w = threadIdx.x >> 6;
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) red[w] = v;
-    __syncthreads();
-    double r = red[0];
-    for (int q = 1; q < kT / 64; ++q) r = nanmax(r, red[q]);
-    return r;
-}
-
-__device__ __forceinline__ void pivot_combine(float &key, int &kp, float ok, int op)
-{
-    const bool take = (ok > key) || (ok == key && op < kp);
-    key = take ? ok : key;
-    kp = take ? op : kp;
-}
-
-struct Smem {
-    // panel / swaps
-    float prow[kPW];
-    float red_key[kT / 64];
-    int red_pos[kT / 64];
-    int swaps[kPW];
-    int src[kPW];
-    int dst[kPW];
-    float L11[kPW][kPW + 1];
-    float T[kPW][kT];          // interchange staging, one column per thread
-    // trailing update operands (fp16 hi/lo), rows/cols x k, k contiguous
-    _Float16 Lh[kTile][kPW + 4], Ll[kTile][kPW + 4];
-    _Float16 Uh[kTile][kPW + 4], Ul[kTile][kPW + 4];
-    // solve / refinement vectors
-    double x[kMaxN];
-    double r[kMaxN];
-    float y[kMaxN];
-    int perm[kMaxN];
-    double red[kT / 64];
+template <int NMAX, int NB, int RG>
+struct MxSmem {
+    union {
+        struct {
+            float blk[NB][NMAX + 1];          // block column, [column][original row]
+            _Float16 Uh[NB][kUPad];           // -U (LU) or -L^T (Cholesky) of the chunk, split
+            _Float16 Ul[NB][kUPad];
+            float L11[CH][CH + 1];
+        } f;
+        struct {
+            double x[RG][NMAX];
+            double r[RG][NMAX];
+            float v[RG][NMAX];
+            float z[RG][CH];
+            float T[CH][CH + 1];              // diagonal block of the sweep
+            double part[kWaves][RG][2];
+        } s;
+    } u;
+    int p[NMAX];          // pivot position -> original row
+    int pivpos[NMAX];     // original row -> pivot position (INT_MAX while unpivoted)
+    int cur[NMAX];        // original row -> current position (LAPACK interchange order)
+    int cexp[NMAX];       // scaling exponents
+    FSlot slot[2][kWaves];
+    float ccol[2][CH];    // Cholesky: the published column of the diagonal block
+    double red[kWaves];
     int flag;
 };
 
-// ---------------------------------------------------------------------------
-// FP32 LU of W (n x n, ld n) with partial pivoting; W holds the LAPACK-layout
-// factors on return, ipiv (1-based) in `ipiv`.  Returns the first zero pivot
-// (1-based) or 0.
-template <int RPT>
-__device__ __forceinline__ int lu32(float *W, int n, int *ipiv, Smem &sm)
+// ---- wavefront reductions (DPP within rows of 16, then the 4 row leaders) ----
+template <int CTRL>
+__device__ __forceinline__ int dpp(int v) { return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xF, 0xF, false); }
+
+__device__ __forceinline__ double dmax_nan(double a, double b) { return (a > b || a != a) ? a : b; }
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v)
 {
-    const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    const int w = tid >> 6;
-    int info = 0;
-    for (int j0 = 0; j0 < n; j0 += kPW) {
-        const int jb = min(kPW, n - j0);
-        const int mp = n - j0;
-        for (int e = tid; e < kPW * (kPW + 1); e += kT) (&sm.L11[0][0])[e] = 0.0f;
-        float a[RPT][kPW];
-        int pos[RPT];
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const int lo = dpp<CTRL>((int)b), hi = dpp<CTRL>((int)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+__device__ __forceinline__ double readlane_d(double v, int l)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// wave-uniform max (NaN wins)
+__device__ __forceinline__ double wave_max(double v)
+{
+    v = dmax_nan(v, dpp_d<0xB1>(v));
+    v = dmax_nan(v, dpp_d<0x4E>(v));
+    v = dmax_nan(v, dpp_d<0x141>(v));
+    v = dmax_nan(v, dpp_d<0x140>(v));
+    double m = readlane_d(v, 0);
 #pragma unroll
-        for (int i = 0; i < RPT; ++i) {
-            const int pr = tid + kT * i;
-            pos[i] = pr;
+    for (int r = 1; r < 4; ++r) m = dmax_nan(m, readlane_d(v, 16 * r));
+    return m;
+}
+
+// arg-max key order: larger key, ties to the smaller current position
+__device__ __forceinline__ void better(float &key, int &cur, float k2, int c2)
+{
+    const bool take = (k2 > key) || (k2 == key && c2 < cur);
+    key = take ? k2 : key;
+    cur = take ? c2 : cur;
+}
+
+template <int CTRL>
+__device__ __forceinline__ void argmax_step(float &key, int &cur)
+{
+    const float k2 = __int_as_float(dpp<CTRL>(__float_as_int(key)));
+    const int c2 = dpp<CTRL>(cur);
+    better(key, cur, k2, c2);
+}
+
+__device__ __forceinline__ void wave_argmax(float &key, int &cur)
+{
+    argmax_step<0xB1>(key, cur);
+    argmax_step<0x4E>(key, cur);
+    argmax_step<0x141>(key, cur);
+    argmax_step<0x140>(key, cur);
+    float k = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(key), 0));
+    int c = __builtin_amdgcn_readlane(cur, 0);
 #pragma unroll
-            for (int c = 0; c < kPW; ++c)
-                a[i][c] = (pr < mp && c < jb) ? W[(j0 + pr) + (size_t)(j0 + c) * n] : 0.0f;
+    for (int r = 1; r < 4; ++r)
+        better(k, c, __int_as_float(__builtin_amdgcn_readlane(__float_as_int(key), 16 * r)),
+               __builtin_amdgcn_readlane(cur, 16 * r));
+    key = k;
+    cur = c;
+}
+
+__device__ __forceinline__ void split(float v, _Float16 &h, _Float16 &l)
+{
+    h = (_Float16)v;
+    l = (_Float16)(v - (float)h);
+}
+
+// ===========================================================================
+template <int NMAX, int NB, int RG>
+struct Mixed {
+    using Smem = MxSmem<NMAX, NB, RG>;
+    static constexpr int RPT = NMAX / kT;     // panel rows per thread
+    static constexpr int RPL = NMAX / 64;     // block-load rows per lane
+
+    // ---- block column load -------------------------------------------------
+    // LU: per-column power-of-two scaling.  Cholesky: symmetric scaling with
+    // the row exponents already in sm.cexp.  Sets sm.flag on a non-finite entry.
+    static __device__ __forceinline__ void load_block(Smem &sm, const double *A, int lda, int n, int c0, int nb, bool chol)
+    {
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll 1
+        for (int cc = w; cc < nb; cc += kWaves) {
+            const int c = c0 + cc;
+            const double *col = A + (size_t)c * lda;
+            double v[RPL];
+            double mx = 0.0;
+            bool bad = false;
+#pragma unroll
+            for (int r = 0; r < RPL; ++r) {
+                const int o = lane + 64 * r;
+                // Cholesky reads the lower part only (upper entries are never used)
+                const bool in = o < n && (!chol || o >= c0);
+                v[r] = in ? col[o] : 0.0;
+                mx = dmax_nan(mx, fabs(v[r]));
+                bad |= !__builtin_isfinite(v[r]);
+            }
+            int e = 0;
+            if (!chol) {
+                mx = wave_max(mx);
+                if (mx > 0.0 && __builtin_isfinite(mx)) frexp(mx, &e);
+            } else {
+                e = sm.cexp[c];
+            }
+            const bool anybad = __builtin_amdgcn_ballot_w64(bad) != 0;
+#pragma unroll
+            for (int r = 0; r < RPL; ++r) {
+                const int o = lane + 64 * r;
+                const int eo = chol ? e + ((o < n) ? sm.cexp[o] : 0) : e;
+                sm.u.f.blk[cc][o] = (float)ldexp(v[r], -eo);
+            }
+            if (lane == 0) {
+                if (!chol) sm.cexp[c] = e;
+                if (anybad) sm.flag = 1;
+            }
         }
-        // ---- panel: pivot search by wave shuffles + LDS combine ----
+    }
+
+    // ---- chunk application ---------------------------------------------------
+    // Applies factor chunk [kq, kq+kb) to block columns [cs, ce).  The chunk's L
+    // comes from the factor storage Fg (global, ld n) or from block column kl0
+    // of the LDS block.  LU: triangular solve on the chunk's pivot rows, then
+    // C -= L U on rows not yet pivoted.  Cholesky: C -= L L^T on rows >= rlo.
+    static __device__ __forceinline__ void apply_chunk(Smem &sm, int n, int c0, int kq, int kb, int cs, int ce, const float *Fg,
+                                       int kl0, bool chol, int rlo)
+    {
+        const int tid = threadIdx.x;
+        if (!chol) {
+            for (int e = tid; e < CH * CH; e += kT) {
+                const int i = e % CH, i2 = e / CH;
+                float v = 0.0f;
+                if (i2 < i && i < kb) {
+                    const int o = sm.p[kq + i];
+                    v = Fg ? Fg[o + (size_t)(kq + i2) * n] : sm.u.f.blk[kl0 + i2][o];
+                }
+                sm.u.f.L11[i][i2] = v;
+            }
+            __syncthreads();
+            // ce - cs <= NB <= kT: one column per thread.  sm.p is a valid row
+            // index at every position, so the loads need no guard.
+            const int cc = cs + tid;
+            if (cc < ce) {
+                // x[j] holds row k + j of the column at step k; the loop over k is
+                // kept rolled (registers rotate instead of being indexed by k)
+                float x[CH];
 #pragma unroll
-        for (int jj = 0; jj < kPW; ++jj) {
-            if (jj < jb) {
-                float key = -1.0f;
-                int kp = INT_MAX;
+                for (int i = 0; i < CH; ++i) {
+                    const float v = sm.u.f.blk[cc][sm.p[min(kq + i, NMAX - 1)]];
+                    x[i] = (i < kb) ? v : 0.0f;
+                }
+#pragma unroll 1
+                for (int k = 0; k < kb; ++k) {
+                    const float xk = x[0];
+                    sm.u.f.blk[cc][sm.p[kq + k]] = xk;
+                    split(-xk, sm.u.f.Uh[cc][k], sm.u.f.Ul[cc][k]);
+                    // rows past the chunk (k + i >= CH) are never output
 #pragma unroll
-                for (int i = 0; i < RPT; ++i) {
-                    const int pr = tid + kT * i;
-                    const bool cand = pr < mp && pos[i] >= jj;
-                    const float v = a[i][jj];
-                    float k2 = -1.0f;
-                    if (cand) k2 = (v != v) ? ((pos[i] == jj) ? __builtin_huge_valf() : -1.0f) : fabsf(v);
-                    pivot_combine(key, kp, k2, cand ? pos[i] : INT_MAX);
+                    for (int i = 1; i < CH; ++i) x[i - 1] = fmaf(sm.u.f.L11[min(k + i, CH - 1)][k], -xk, x[i]);
+                    x[CH - 1] = 0.0f;
+                }
+                for (int k = kb; k < CH; ++k) {
+                    sm.u.f.Uh[cc][k] = (_Float16)0.0f;
+                    sm.u.f.Ul[cc][k] = (_Float16)0.0f;
+                }
+            }
+        } else {
+            const int w = ce - cs;
+            for (int e = tid; e < w * CH; e += kT) {
+                const int cc = cs + e % w, k = e / w;
+                float v = 0.0f;
+                if (k < kb) v = Fg ? Fg[(c0 + cc) + (size_t)(kq + k) * n] : sm.u.f.blk[kl0 + k][c0 + cc];
+                split(-v, sm.u.f.Uh[cc][k], sm.u.f.Ul[cc][k]);
+            }
+        }
+        __syncthreads();
+        gemm(sm, n, kq, kb, cs, ce, Fg, kl0, rlo);
+        __syncthreads();
+    }
+
+    // blk[c][o] -= sum_k L(o, kq+k) * U(k, c) for c in [cs, ce) and rows o >= rlo
+    // with pivpos[o] >= kq + kb; -U^T is staged in Uh/Ul.
+    static __device__ __forceinline__ void gemm(Smem &sm, int n, int kq, int kb, int cs, int ce, const float *Fg, int kl0, int rlo)
+    {
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        const int li = lane & 15, lk = lane >> 4;
+        const int ostart = (rlo / 16) * 16;
+#pragma unroll 1
+        for (int o0 = ostart + 16 * w; o0 < n; o0 += 16 * kWaves) {
+            const int o = o0 + li;
+            const bool act = o < n && o >= rlo && sm.pivpos[o] >= kq + kb;
+            half4 lh[2], ll[2];
+#pragma unroll
+            for (int ks = 0; ks < 2; ++ks) {
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int k = 16 * ks + 4 * lk + t;
+                    float v = 0.0f;
+                    if (act && k < kb) v = Fg ? Fg[o + (size_t)(kq + k) * n] : sm.u.f.blk[kl0 + k][o];
+                    _Float16 h, l;
+                    split(v, h, l);
+                    lh[ks][t] = h;
+                    ll[ks][t] = l;
+                }
+            }
+#pragma unroll 1
+            for (int cb = cs; cb < ce; cb += 16) {
+                float4v acc;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int c = cb + 4 * lk + i;
+                    acc[i] = (c < ce) ? sm.u.f.blk[c][o] : 0.0f;
                 }
 #pragma unroll
-                for (int off = 32; off > 0; off >>= 1)
-                    pivot_combine(key, kp, __shfl_xor(key, off), __shfl_xor(kp, off));
-                if (lane == 0) {
-                    sm.red_key[w] = key;
-                    sm.red_pos[w] = kp;
+                for (int ks = 0; ks < (CH / 16); ++ks) {
+                    const half4 uh = *reinterpret_cast<const half4 *>(&sm.u.f.Uh[cb + li][16 * ks + 4 * lk]);
+                    const half4 ul = *reinterpret_cast<const half4 *>(&sm.u.f.Ul[cb + li][16 * ks + 4 * lk]);
+                    acc = __builtin_amdgcn_mfma_f32_16x16x16f16(ul, lh[ks], acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_16x16x16f16(uh, ll[ks], acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_16x16x16f16(uh, lh[ks], acc, 0, 0, 0);
                 }
-                __syncthreads();
-                key = sm.red_key[0];
-                kp = sm.red_pos[0];
+                if (act) {
 #pragma unroll
-                for (int q = 1; q < kT / 64; ++q) pivot_combine(key, kp, sm.red_key[q], sm.red_pos[q]);
-                const int p = kp;
-#pragma unroll
-                for (int i = 0; i < RPT; ++i) {
-                    const int pr = tid + kT * i;
-                    if (pr < mp && pos[i] == p) {
-#pragma unroll
-                        for (int c = jj; c < kPW; ++c) sm.prow[c] = a[i][c];
+                    for (int i = 0; i < 4; ++i) {
+                        const int c = cb + 4 * lk + i;
+                        if (c < ce) sm.u.f.blk[c][o] = acc[i];
                     }
                 }
-                if (tid == 0) sm.swaps[jj] = p;
+            }
+        }
+    }
+
+    // ---- LU chunk panel: block columns [s0, s0+kb), global columns c0+s0.. ----
+    // Returns nonzero on a zero or NaN pivot.
+    static __device__ __forceinline__ int panel_lu(Smem &sm, int n, int c0, int s0, int kb, int *ipiv)
+    {
+        const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+        float a[RPT][CH];
+        int cur[RPT];
+        bool act[RPT];
+        int bad = 0;
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            const int o = tid + kT * r;
+            act[r] = o < n && sm.pivpos[o] == INT_MAX;
+            cur[r] = sm.cur[o];
+#pragma unroll
+            for (int cc = 0; cc < CH; ++cc) a[r][cc] = (cc < kb) ? sm.u.f.blk[s0 + cc][o] : 0.0f;
+        }
+#pragma unroll
+        for (int jj = 0; jj < CH; ++jj) {
+            if (jj < kb) {
+                const int gpos = c0 + s0 + jj;
+                float key = -1.0f;
+                int kc = INT_MAX, own = -1;
+#pragma unroll
+                for (int r = 0; r < RPT; ++r) {
+                    if (act[r]) {
+                        const float v = a[r][jj];
+                        const float k2 = (v != v) ? __builtin_huge_valf() : fabsf(v);
+                        const bool take = (k2 > key) || (k2 == key && cur[r] < kc);
+                        key = take ? k2 : key;
+                        kc = take ? cur[r] : kc;
+                        own = take ? r : own;
+                    }
+                }
+                float wk = key;
+                int wc = kc;
+                wave_argmax(wk, wc);
+                FSlot &sl = sm.slot[jj & 1][w];
+                if (own >= 0 && kc == wc) {
+#pragma unroll
+                    for (int cc = jj; cc < CH; ++cc) {
+                        float v = a[0][cc];
+#pragma unroll
+                        for (int r = 1; r < RPT; ++r) v = (own == r) ? a[r][cc] : v;
+                        sl.val[cc] = v;
+                    }
+                    sl.row = tid + kT * own;
+                }
+                if (lane == 0) {
+                    sl.key = wk;
+                    sl.cur = wc;
+                }
                 __syncthreads();
-                const float piv = sm.prow[jj];
-                if (piv == 0.0f && info == 0) info = j0 + jj + 1;
+                float bk = sm.slot[jj & 1][0].key;
+                int bc = sm.slot[jj & 1][0].cur, bw = 0;
+#pragma unroll
+                for (int q = 1; q < kWaves; ++q) {
+                    const float k2 = sm.slot[jj & 1][q].key;
+                    const int c2 = sm.slot[jj & 1][q].cur;
+                    const bool take = (k2 > bk) || (k2 == bk && c2 < bc);
+                    bk = take ? k2 : bk;
+                    bc = take ? c2 : bc;
+                    bw = take ? q : bw;
+                }
+                const FSlot &ps = sm.slot[jj & 1][bw];
+                const int prow = ps.row;
+                const float piv = ps.val[jj];
+                if (!(piv != 0.0f) || piv != piv) bad = 1;
+                if (tid == 0) {
+                    sm.p[gpos] = prow;
+                    if (ipiv) ipiv[gpos] = bc + 1;
+                }
                 const float rc = 1.0f / piv;
 #pragma unroll
-                for (int i = 0; i < RPT; ++i) {
-                    const int pr = tid + kT * i;
-                    pos[i] = (pos[i] == jj) ? p : ((pos[i] == p) ? jj : pos[i]);
-                    if (pr < mp && pos[i] > jj && piv != 0.0f) {
-                        const float l = a[i][jj] * rc;
-                        a[i][jj] = l;
+                for (int r = 0; r < RPT; ++r) {
+                    const int o = tid + kT * r;
+                    const bool isp = (o == prow);
+                    if (isp) {
+                        act[r] = false;
+                        sm.pivpos[o] = gpos;
+                    }
+                    cur[r] = isp ? gpos : ((cur[r] == gpos) ? bc : cur[r]);
+                    if (act[r] && !bad) {
+                        const float l = a[r][jj] * rc;
+                        a[r][jj] = l;
 #pragma unroll
-                        for (int c = jj + 1; c < kPW; ++c) a[i][c] = fmaf(l, -sm.prow[c], a[i][c]);
+                        for (int cc = jj + 1; cc < CH; ++cc) a[r][cc] = fmaf(l, -ps.val[cc], a[r][cc]);
                     }
                 }
             }
         }
-        // ---- panel back in pivoted order; L11 and the row mapping to LDS ----
-        if (tid < kPW) sm.dst[tid] = -1;
-        __syncthreads();
 #pragma unroll
-        for (int i = 0; i < RPT; ++i) {
-            const int pr = tid + kT * i;
-            if (pr < mp) {
+        for (int r = 0; r < RPT; ++r) {
+            const int o = tid + kT * r;
 #pragma unroll
-                for (int c = 0; c < kPW; ++c)
-                    if (c < jb) W[(j0 + pos[i]) + (size_t)(j0 + c) * n] = a[i][c];
-                if (pos[i] < jb) {
-                    sm.src[pos[i]] = pr;
-#pragma unroll
-                    for (int c = 0; c < kPW; ++c) sm.L11[pos[i]][c] = a[i][c];
-                }
-                if (pr < jb && pos[i] >= jb) sm.dst[pr] = pos[i];
-            }
-        }
-        if (tid < jb) ipiv[j0 + tid] = j0 + sm.swaps[tid] + 1;
-        __syncthreads();
-        // ---- interchanges on the other columns (all loads first, then the
-        //      stores), and U12 = L11^-1 A12 on the right ones ----
-        for (int t0 = 0; t0 < n - jb; t0 += kT) {
-            const int t = t0 + tid;
-            const bool act = t < n - jb;
-            const int c = (t < j0) ? t : t + jb;
-            float *col = W + (size_t)c * n + j0;
-            // old top rows of this column to LDS (one column per thread)
-            if (act) {
-#pragma unroll
-                for (int k = 0; k < kPW; ++k)
-                    if (k < jb) sm.T[k][tid] = col[k];
-            }
-            float nw[kPW];
-#pragma unroll
-            for (int k = 0; k < kPW; ++k) nw[k] = 0.0f;
-            if (act) {
-#pragma unroll
-                for (int k = 0; k < kPW; ++k) {
-                    if (k < jb) {
-                        const int s = sm.src[k];
-                        nw[k] = (s < jb) ? sm.T[s][tid] : col[s];
-                    }
-                }
-#pragma unroll
-                for (int k = 0; k < kPW; ++k)
-                    if (k < jb && sm.dst[k] >= jb) col[sm.dst[k]] = sm.T[k][tid];
-                if (c >= j0 + jb) {
-#pragma unroll
-                    for (int k = 0; k < kPW; ++k) {
-                        const float u = -nw[k];
-#pragma unroll
-                        for (int i = k + 1; i < kPW; ++i) nw[i] = fmaf(sm.L11[i][k], u, nw[i]);
-                        __builtin_amdgcn_sched_barrier(0);
-                    }
-                }
-#pragma unroll
-                for (int k = 0; k < kPW; ++k)
-                    if (k < jb) col[k] = nw[k];
-            }
+            for (int cc = 0; cc < CH; ++cc)
+                if (cc < kb) sm.u.f.blk[s0 + cc][o] = a[r][cc];
+            sm.cur[o] = cur[r];
         }
         __syncthreads();
-        // ---- trailing update on FP16 MFMA, split operands ----
-        const int r0 = j0 + jb, c0 = j0 + jb;
-        if (r0 < n) {
-            const int wr = (w & 1) * 32, wc = (w >> 1) * 32;     // wave quadrant of the 64x64 tile
-            const int li = lane & 15, lq = lane >> 4;
-            for (int tc = c0; tc < n; tc += kTile) {
-                for (int tr = r0; tr < n; tr += kTile) {
-                    // stage L21 rows tr..tr+63 and U12 columns tc..tc+63 (k = jb), split
-                    for (int e = tid; e < kTile * kPW; e += kT) {
-                        const int i = e % kTile, k = e / kTile;
-                        const int gi = tr + i;
-                        const float v = (gi < n && k < jb) ? -W[gi + (size_t)(j0 + k) * n] : 0.0f;
-                        const _Float16 h = (_Float16)v;
-                        sm.Lh[i][k] = h;
-                        sm.Ll[i][k] = (_Float16)(v - (float)h);
-                    }
-                    for (int e = tid; e < kTile * kPW; e += kT) {
-                        const int k = e % kPW, c = e / kPW;
-                        const int gc = tc + c;
-                        const float v = (gc < n && k < jb) ? W[(j0 + k) + (size_t)gc * n] : 0.0f;
-                        const _Float16 h = (_Float16)v;
-                        sm.Uh[c][k] = h;
-                        sm.Ul[c][k] = (_Float16)(v - (float)h);
-                    }
-                    __syncthreads();
-                    float4v acc[2][2];
-#pragma unroll
-                    for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-                        for (int bj = 0; bj < 2; ++bj) {
-                            const int col = tc + wc + 16 * bj + li;
-#pragma unroll
-                            for (int q = 0; q < 4; ++q) {
-                                const int row = tr + wr + 16 * bi + 4 * lq + q;
-                                acc[bi][bj][q] = (row < n && col < n) ? W[row + (size_t)col * n] : 0.0f;
-                            }
-                        }
-#pragma unroll
-                    for (int ks = 0; ks < kPW; ks += 16) {
-                        half4 ah[2], al[2], bh[2], bl[2];
-#pragma unroll
-                        for (int bi = 0; bi < 2; ++bi) {
-                            const int i = wr + 16 * bi + li;
-                            ah[bi] = *reinterpret_cast<const half4 *>(&sm.Lh[i][ks + 4 * lq]);
-                            al[bi] = *reinterpret_cast<const half4 *>(&sm.Ll[i][ks + 4 * lq]);
-                        }
-#pragma unroll
-                        for (int bj = 0; bj < 2; ++bj) {
-                            const int c = wc + 16 * bj + li;
-                            bh[bj] = *reinterpret_cast<const half4 *>(&sm.Uh[c][ks + 4 * lq]);
-                            bl[bj] = *reinterpret_cast<const half4 *>(&sm.Ul[c][ks + 4 * lq]);
-                        }
-#pragma unroll
-                        for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-                            for (int bj = 0; bj < 2; ++bj) {
-                                acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x16f16(al[bi], bh[bj], acc[bi][bj], 0, 0, 0);
-                                acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah[bi], bl[bj], acc[bi][bj], 0, 0, 0);
-                                acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah[bi], bh[bj], acc[bi][bj], 0, 0, 0);
-                            }
-                    }
-#pragma unroll
-                    for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-                        for (int bj = 0; bj < 2; ++bj) {
-                            const int col = tc + wc + 16 * bj + li;
-#pragma unroll
-                            for (int q = 0; q < 4; ++q) {
-                                const int row = tr + wr + 16 * bi + 4 * lq + q;
-                                if (row < n && col < n) W[row + (size_t)col * n] = acc[bi][bj][q];
-                            }
-                        }
-                    __syncthreads();
-                }
-            }
-        }
+        return bad;
     }
-    return info;
-}
 
-// y (LDS, FP32, length n) := U^-1 L^-1 y, blocked by 32: a wavefront solves
-// each diagonal block with shuffles, all threads apply the block to the rest.
-__device__ __forceinline__ void trsv32(const float *W, int n, float *y)
-{
-    const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    // forward, unit lower
-    for (int k0 = 0; k0 < n; k0 += 32) {
-        const int kb = min(32, n - k0);
-        if (tid < 64) {
-            float v = (lane < kb) ? y[k0 + lane] : 0.0f;
-            float row[32];
+    // ---- Cholesky chunk panel: columns [s0, s0+kb) of the block, rows >= c0+s0 ----
+    static __device__ __forceinline__ int panel_chol(Smem &sm, int n, int c0, int s0, int kb)
+    {
+        const int tid = threadIdx.x;
+        const int d0 = c0 + s0;          // first row / column of the diagonal block
+        float a[RPT][CH];
+        int bad = 0;
 #pragma unroll
-            for (int k = 0; k < 32; ++k) row[k] = (lane < kb && k < kb) ? W[(k0 + lane) + (size_t)(k0 + k) * n] : 0.0f;
+        for (int r = 0; r < RPT; ++r) {
+            const int o = tid + kT * r;
 #pragma unroll
-            for (int k = 0; k < 32; ++k) {
-                const float yk = __shfl(v, k);
-                if (lane > k) v = fmaf(-row[k], yk, v);
-            }
-            if (lane < kb) y[k0 + lane] = v;
+            for (int cc = 0; cc < CH; ++cc) a[r][cc] = (cc < kb && o >= d0 && o < n) ? sm.u.f.blk[s0 + cc][o] : 0.0f;
         }
-        __syncthreads();
-        for (int r = k0 + kb + tid; r < n; r += kT) {
-            float v = y[r];
-            for (int k = 0; k < kb; ++k) v = fmaf(-W[r + (size_t)(k0 + k) * n], y[k0 + k], v);
-            y[r] = v;
-        }
-        __syncthreads();
-    }
-    // backward, upper
-    for (int kend = n; kend > 0; kend -= 32) {
-        const int k0 = max(0, kend - 32);
-        const int kb = kend - k0;
-        if (tid < 64) {
-            float v = (lane < kb) ? y[k0 + lane] : 0.0f;
-            float row[32];
 #pragma unroll
-            for (int k = 0; k < 32; ++k) row[k] = (lane < kb && k < kb) ? W[(k0 + lane) + (size_t)(k0 + k) * n] : 1.0f;
+        for (int jj = 0; jj < CH; ++jj) {
+            if (jj < kb) {
 #pragma unroll
-            for (int k = 31; k >= 0; --k) {
-                if (k < kb) {
-                    if (lane == k) v = v / row[k];
-                    const float xk = __shfl(v, k);
-                    if (lane < k) v = fmaf(-row[k], xk, v);
+                for (int r = 0; r < RPT; ++r) {
+                    const int i = tid + kT * r - d0;
+                    if (i >= jj && i < kb) sm.ccol[jj & 1][i] = a[r][jj];
+                }
+                __syncthreads();
+                const float d = sm.ccol[jj & 1][jj];
+                if (!(d > 0.0f)) bad = 1;
+                const float ljj = sqrtf(d);
+                const float rl = 1.0f / ljj;
+#pragma unroll
+                for (int r = 0; r < RPT; ++r) {
+                    const int o = tid + kT * r;
+                    if (o == d0 + jj) a[r][jj] = ljj;
+                    if (o > d0 + jj && o < n && !bad) {
+                        const float l = a[r][jj] * rl;
+                        a[r][jj] = l;
+#pragma unroll
+                        for (int cc = jj + 1; cc < CH; ++cc)
+                            if (cc < kb) a[r][cc] = fmaf(-l, sm.ccol[jj & 1][cc] * rl, a[r][cc]);
+                    }
                 }
             }
-            if (lane < kb) y[k0 + lane] = v;
+        }
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            const int o = tid + kT * r;
+            if (o >= d0 && o < n) {
+#pragma unroll
+                for (int cc = 0; cc < CH; ++cc)
+                    if (cc < kb) sm.u.f.blk[s0 + cc][o] = a[r][cc];
+            }
         }
         __syncthreads();
-        for (int r = tid; r < k0; r += kT) {
-            float v = y[r];
-            for (int k = 0; k < kb; ++k) v = fmaf(-W[r + (size_t)(k0 + k) * n], y[k0 + k], v);
-            y[r] = v;
+        return bad;
+    }
+
+    // ---- block column to the factor storage ----------------------------------
+    static __device__ __forceinline__ void write_block(Smem &sm, float *F, int n, int c0, int nb, bool chol)
+    {
+        const int tid = threadIdx.x;
+        if (!chol) {
+            for (int e = tid; e < nb * n; e += kT) {
+                const int cc = e / n, o = e % n;
+                F[o + (size_t)(c0 + cc) * n] = sm.u.f.blk[cc][o];
+            }
+        } else {
+            // lower part in place; the transpose into the upper part (used by the
+            // backward sweep as an upper-triangular factor)
+            for (int e = tid; e < nb * n; e += kT) {
+                const int cc = e / n, o = e % n;
+                if (o >= c0 + cc) F[o + (size_t)(c0 + cc) * n] = sm.u.f.blk[cc][o];
+            }
+            for (int e = tid; e < nb * n; e += kT) {
+                const int cc = e % nb, o = e / nb;
+                if (o > c0 + cc) F[(c0 + cc) + (size_t)o * n] = sm.u.f.blk[cc][o];
+            }
+        }
+    }
+
+    // ---- the whole FP32 factorization; returns the status code --------------
+    static __device__ __forceinline__ int factor(Smem &sm, const double *A, int lda, int n, float *F, int *ipiv, bool chol)
+    {
+        const int tid = threadIdx.x;
+        for (int o = tid; o < NMAX; o += kT) {
+            sm.pivpos[o] = chol ? o : ((o < n) ? INT_MAX : -1);
+            sm.p[o] = o;
+            sm.cur[o] = o;
+        }
+        if (tid == 0) sm.flag = 0;
+        __syncthreads();
+        if (chol) {
+            // symmetric scaling from the diagonal: a_ii 2^-2e_i in [0.5, 2)
+            for (int o = tid; o < n; o += kT) {
+                const double d = A[o + (size_t)o * lda];
+                int e = 0;
+                if (d > 0.0 && __builtin_isfinite(d)) {
+                    frexp(d, &e);
+                    e >>= 1;
+                } else {
+                    sm.flag = (d != d || !__builtin_isfinite(d)) ? 1 : 2;
+                }
+                sm.cexp[o] = e;
+            }
+        }
+        __syncthreads();
+        const int f0 = sm.flag;
+        __syncthreads();
+        if (f0) return f0 == 1 ? -2 : -3;
+        int status = 0;
+#pragma unroll 1
+        for (int c0 = 0; c0 < n; c0 += NB) {
+            const int nb = min(NB, n - c0);
+            load_block(sm, A, lda, n, c0, nb, chol);
+            __syncthreads();
+            if (sm.flag) {
+                status = -2;
+                break;
+            }
+            const int rlo = chol ? c0 : 0;
+#pragma unroll 1
+            for (int kq = 0; kq < c0; kq += CH) apply_chunk(sm, n, c0, kq, CH, 0, nb, F, -1, chol, rlo);
+#pragma unroll 1
+            for (int s0 = 0; s0 < nb; s0 += CH) {
+                const int kb = min(CH, nb - s0);
+                const int bad = chol ? panel_chol(sm, n, c0, s0, kb) : panel_lu(sm, n, c0, s0, kb, ipiv);
+                if (bad && status == 0) status = -3;
+                if (s0 + kb < nb) apply_chunk(sm, n, c0, c0 + s0, kb, s0 + kb, nb, nullptr, s0, chol,
+                                                     chol ? c0 + s0 + kb : 0);
+            }
+            write_block(sm, F, n, c0, nb, chol);
+            __syncthreads();
+        }
+        int *Fexp = reinterpret_cast<int *>(F + (size_t)n * n);
+        for (int o = tid; o < n; o += kT) {
+            Fexp[o] = sm.cexp[o];
+            Fexp[n + o] = sm.p[o];
+        }
+        return status;
+    }
+
+    // pivot order and exponents of a stored factorization
+    static __device__ __forceinline__ void load_factor_meta(Smem &sm, const float *F, int n, bool chol)
+    {
+        const int tid = threadIdx.x;
+        const int *Fexp = reinterpret_cast<const int *>(F + (size_t)n * n);
+        for (int o = tid; o < n; o += kT) {
+            sm.cexp[o] = Fexp[o];
+            const int po = chol ? o : Fexp[n + o];
+            sm.p[o] = po;
+            sm.pivpos[po] = o;
         }
         __syncthreads();
     }
-}
 
-template <int RPT>
-__global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(2))) dsgesv_kernel(MixedArgs g)
-{
-    __shared__ Smem sm;
-    const int tid = threadIdx.x;
-    const int n = g.n;
-    float *W = g.W + (size_t)blockIdx.x * n * n;
-    const double eps = 0x1p-53;
-    for (int item = blockIdx.x; item < g.count; item += gridDim.x) {
-        const double *A = g.A[item];
-        const double *B = g.B[item];
-        double *X = g.X[item];
-        int *ipiv = g.ipiv[item];
-        const int lda = g.lda;
-        // ---- 1. norms and scaled FP32 copy ----
-        double rowsum_max = 0.0, amax = 0.0;
-        for (int r = tid; r < n; r += kT) {
-            double s = 0.0, m = 0.0;
-            for (int c = 0; c < n; ++c) {
-                const double v = fabs(A[r + (size_t)c * lda]);
-                s += v;
-                m = nanmax(m, v);
-            }
-            rowsum_max = nanmax(rowsum_max, s);
-            amax = nanmax(amax, m);
-        }
-        const double anrm = block_max(rowsum_max, sm.red);
-        const double amx = block_max(amax, sm.red);
-        int status = 0;           // 0 ok, -2 unscalable, -3 zero pivot, -(ITERMAX+1) no convergence
-        if (!(amx > 0.0) || !__builtin_isfinite(amx)) {
-            status = (amx == 0.0) ? -3 : -2;
-        }
-        int ex = 0;
-        if (status == 0) {
-            frexp(amx, &ex);
-            const double sc = ldexp(1.0, -ex);
-            for (int e = tid; e < n * n; e += kT) {
-                const int r = e % n, c = e / n;
-                W[e] = (float)(A[r + (size_t)c * lda] * sc);
+    // ---- correction solve: v (FP32, original-row order) := (LU)^-1 v ---------
+    // forward sweep with L (unit for LU), backward sweep with U (for Cholesky
+    // the transposed factor stored above the diagonal)
+    static __device__ __forceinline__ void sweeps(Smem &sm, const float *F, int n, int nr, bool chol)
+    {
+        const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+        // forward
+#pragma unroll 1
+        for (int kq = 0; kq < n; kq += CH) {
+            const int kb = min(CH, n - kq);
+            for (int e = tid; e < CH * CH; e += kT) {
+                const int i = e % CH, k = e / CH;
+                float v = 0.0f;
+                if (i < kb && k < kb && k <= i) v = F[sm.p[kq + i] + (size_t)(kq + k) * n];
+                sm.u.s.T[i][k] = v;
             }
             __syncthreads();
-            // ---- 2. FP32 LU ----
-            if (lu32<RPT>(W, n, ipiv, sm) != 0) status = -3;
+            for (int g = w; g < nr; g += kWaves) {
+                const float v0 = sm.u.s.v[g][sm.p[min(kq + lane, NMAX - 1)]];
+                float vi = (lane < kb) ? v0 : 0.0f;
+#pragma unroll
+                for (int k = 0; k < CH; ++k) {
+                    if (k < kb) {
+                        if (chol && lane == k) vi = vi / sm.u.s.T[k][k];
+                        const float zk = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(vi), k));
+                        if (lane > k && lane < kb) vi = fmaf(-sm.u.s.T[lane & (CH - 1)][k], zk, vi);
+                    }
+                }
+                if (lane < kb) {
+                    sm.u.s.z[g][lane] = vi;
+                    sm.u.s.v[g][sm.p[kq + lane]] = vi;
+                }
+            }
+            __syncthreads();
+            if (kq + kb < n) {
+                for (int o = tid; o < n; o += kT) {
+                    if (sm.pivpos[o] >= kq + kb) {
+                        float f[CH];
+#pragma unroll
+                        for (int k = 0; k < CH; ++k) f[k] = (k < kb) ? F[o + (size_t)(kq + k) * n] : 0.0f;
+                        for (int g = 0; g < nr; ++g) {
+                            float acc = sm.u.s.v[g][o];
+#pragma unroll
+                            for (int k = 0; k < CH; ++k) acc = fmaf(-f[k], sm.u.s.z[g][k], acc);
+                            sm.u.s.v[g][o] = acc;
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        // backward
+#pragma unroll 1
+        for (int kend = n; kend > 0; kend -= CH) {
+            const int kq = max(0, kend - CH);
+            const int kb = kend - kq;
+            for (int e = tid; e < CH * CH; e += kT) {
+                const int i = e % CH, k = e / CH;
+                float v = 0.0f;
+                if (i < kb && k < kb && k >= i) v = F[sm.p[kq + i] + (size_t)(kq + k) * n];
+                sm.u.s.T[i][k] = v;
+            }
+            __syncthreads();
+            for (int g = w; g < nr; g += kWaves) {
+                const float v0 = sm.u.s.v[g][sm.p[min(kq + lane, NMAX - 1)]];
+                float vi = (lane < kb) ? v0 : 0.0f;
+#pragma unroll
+                for (int k = CH - 1; k >= 0; --k) {
+                    if (k < kb) {
+                        if (lane == k) vi = vi / sm.u.s.T[k][k];
+                        const float yk = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(vi), k));
+                        if (lane < k) vi = fmaf(-sm.u.s.T[lane & (CH - 1)][k], yk, vi);
+                    }
+                }
+                if (lane < kb) {
+                    sm.u.s.z[g][lane] = vi;
+                    sm.u.s.v[g][sm.p[kq + lane]] = vi;
+                }
+            }
+            __syncthreads();
+            if (kq > 0) {
+                for (int o = tid; o < n; o += kT) {
+                    if (sm.pivpos[o] < kq) {
+                        float f[CH];
+#pragma unroll
+                        for (int k = 0; k < CH; ++k) f[k] = (k < kb) ? F[o + (size_t)(kq + k) * n] : 0.0f;
+                        for (int g = 0; g < nr; ++g) {
+                            float acc = sm.u.s.v[g][o];
+#pragma unroll
+                            for (int k = 0; k < CH; ++k) acc = fmaf(-f[k], sm.u.s.z[g][k], acc);
+                            sm.u.s.v[g][o] = acc;
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+
+    // ---- refinement of one group of right-hand sides; returns iters or status --
+    static __device__ __forceinline__ int refine(Smem &sm, const double *A, int lda, const double *B, int ldb, double *X, int ldx,
+                                 const float *F, int n, int nr, bool chol, double &anrm, double &berr)
+    {
+        const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+        const double eps = 0x1p-53;
+        for (int e = tid; e < RG * NMAX; e += kT) {
+            const int g = e / NMAX, o = e % NMAX;
+            sm.u.s.x[g][o] = 0.0;
+            sm.u.s.r[g][o] = (g < nr && o < n) ? B[o + (size_t)g * ldb] : 0.0;
+        }
+        __syncthreads();
+        int result = -(MAGMA_MIXED_ITERMAX + 1);
+#pragma unroll 1
+        for (int it = 0; it <= MAGMA_MIXED_ITERMAX; ++it) {
+            // v = r (Cholesky: D r), FP32
+            for (int e = tid; e < nr * NMAX; e += kT) {
+                const int g = e / NMAX, o = e % NMAX;
+                float v = 0.0f;
+                if (o < n) v = (float)(chol ? ldexp(sm.u.s.r[g][o], -sm.cexp[o]) : sm.u.s.r[g][o]);
+                sm.u.s.v[g][o] = v;
+            }
+            __syncthreads();
+            sweeps(sm, F, n, nr, chol);
+            // x += D y, with y_c held at row p(c)
+            for (int e = tid; e < nr * NMAX; e += kT) {
+                const int g = e / NMAX, c = e % NMAX;
+                if (c < n) sm.u.s.x[g][c] += ldexp((double)sm.u.s.v[g][sm.p[c]], -sm.cexp[c]);
+            }
+            __syncthreads();
+            // r = b - A x (FP64); ||A||_inf on the first pass
+            double rmax[RG], xmax[RG];
+#pragma unroll
+            for (int g = 0; g < RG; ++g) {
+                rmax[g] = 0.0;
+                xmax[g] = 0.0;
+            }
+            double rs_max = 0.0;
+            for (int o = tid; o < n; o += kT) {
+                double acc[RG];
+#pragma unroll
+                for (int g = 0; g < RG; ++g) acc[g] = (g < nr) ? B[o + (size_t)g * ldb] : 0.0;
+                double rs = 0.0;
+                const double *arow = A + o;
+                for (int c = 0; c < n; ++c) {
+                    const double a = arow[(size_t)c * lda];
+                    rs += fabs(a);
+#pragma unroll
+                    for (int g = 0; g < RG; ++g)
+                        if (g < nr) acc[g] = fma(-a, sm.u.s.x[g][c], acc[g]);
+                }
+                rs_max = dmax_nan(rs_max, rs);
+#pragma unroll
+                for (int g = 0; g < RG; ++g) {
+                    if (g < nr) {
+                        sm.u.s.r[g][o] = acc[g];
+                        rmax[g] = dmax_nan(rmax[g], fabs(acc[g]));
+                        xmax[g] = dmax_nan(xmax[g], fabs(sm.u.s.x[g][o]));
+                    }
+                }
+            }
+            // workgroup maxima
+            rs_max = wave_max(rs_max);
+#pragma unroll
+            for (int g = 0; g < RG; ++g) {
+                if (g < nr) {
+                    rmax[g] = wave_max(rmax[g]);
+                    xmax[g] = wave_max(xmax[g]);
+                }
+            }
+            if (lane == 0) {
+                sm.red[w] = rs_max;
+#pragma unroll
+                for (int g = 0; g < RG; ++g) {
+                    sm.u.s.part[w][g][0] = rmax[g];
+                    sm.u.s.part[w][g][1] = xmax[g];
+                }
+            }
+            __syncthreads();
+            if (it == 0) {
+                double m = sm.red[0];
+#pragma unroll
+                for (int q = 1; q < kWaves; ++q) m = dmax_nan(m, sm.red[q]);
+                anrm = dmax_nan(anrm, m);
+            }
+            const double cte = anrm * eps * sqrt((double)n);
+            bool conv = true;
+            double be = 0.0;
+            for (int g = 0; g < nr; ++g) {
+                double rn = sm.u.s.part[0][g][0], xn = sm.u.s.part[0][g][1];
+#pragma unroll
+                for (int q = 1; q < kWaves; ++q) {
+                    rn = dmax_nan(rn, sm.u.s.part[q][g][0]);
+                    xn = dmax_nan(xn, sm.u.s.part[q][g][1]);
+                }
+                if (!(rn <= xn * cte)) conv = false;
+                be = dmax_nan(be, rn / (anrm * xn));
+            }
+            __syncthreads();
+            berr = be;
+            if (conv) {
+                result = it;
+                break;
+            }
+        }
+        if (result >= 0) {
+            for (int e = tid; e < nr * n; e += kT) {
+                const int g = e / n, o = e % n;
+                X[o + (size_t)g * ldx] = sm.u.s.x[g][o];
+            }
+        }
+        __syncthreads();
+        return result;
+    }
+};
+
+template <int NMAX, int NB, int RG>
+__global__ void __launch_bounds__(kT) mixed_kernel(MixedSet g)
+{
+    using M = Mixed<NMAX, NB, RG>;
+    __shared__ typename M::Smem sm;
+    const int tid = threadIdx.x;
+    for (int item = blockIdx.x; item < g.count; item += gridDim.x) {
+        const int id = g.index ? g.index[item] : item;
+        const int n = pick(g.n_arr, g.n, id);
+        const int nrhs = pick(g.nrhs_arr, g.nrhs, id);
+        const int lda = pick(g.lda_arr, g.lda, id);
+        const int ldb = pick(g.ldb_arr, g.ldb, id);
+        const int ldx = pick(g.ldx_arr, g.ldx, id);
+        const bool chol = (g.kind_arr ? g.kind_arr[id] : g.kind) == 'C';
+        if (n <= 0 || n > NMAX) {
+            if (tid == 0) {
+                if (g.iter) g.iter[id] = 0;
+                if (g.info) g.info[id] = 0;
+            }
+            continue;
+        }
+        float *F = g.F ? g.F[id] : g.Wslot + g.slot_floats * blockIdx.x;
+        const double *A = g.A[id];
+        int status = 0;
+        if (g.reuse && g.reuse[id]) {
+            M::load_factor_meta(sm, F, n, chol);
+        } else {
+            status = M::factor(sm, A, lda, n, F, (!chol && g.ipiv) ? g.ipiv[id] : nullptr, chol);
+            __syncthreads();
         }
         int iters = 0;
-        if (status == 0) {
-            // composed row permutation of the factorization: row i of P b is b[perm[i]]
-            for (int i = tid; i < n; i += kT) sm.perm[i] = i;
-            __syncthreads();
-            if (tid == 0)
-                for (int k = 0; k < n; ++k) {
-                    const int p = ipiv[k] - 1;
-                    const int t = sm.perm[k];
-                    sm.perm[k] = sm.perm[p];
-                    sm.perm[p] = t;
-                }
-            __syncthreads();
-            const double cte = anrm * eps * sqrt((double)n);   // BWDMAX = 1
-            const double inv = ldexp(1.0, -ex);                // A = 2^ex * (factored matrix)
-            // columns are refined one after the other; iter is the largest
-            // number of correction sweeps any column needed
-            for (int j = 0; j < g.nrhs && status == 0; ++j) {
-                const double *b = B + (size_t)j * g.ldb;
-                double *x = X + (size_t)j * g.ldx;
-                for (int i = tid; i < n; i += kT) {
-                    sm.x[i] = 0.0;
-                    sm.r[i] = b[i];
-                }
-                __syncthreads();
-                bool conv = false;
-                for (int it = 0; it <= MAGMA_MIXED_ITERMAX; ++it) {
-                    // d = LU32^-1 (P r) in FP32, x += d in FP64
-                    for (int i = tid; i < n; i += kT) sm.y[i] = (float)(sm.r[sm.perm[i]] * inv);
-                    __syncthreads();
-                    trsv32(W, n, sm.y);
-                    for (int i = tid; i < n; i += kT) sm.x[i] += (double)sm.y[i];
-                    __syncthreads();
-                    // r = b - A x in FP64
-                    double rmax = 0.0, xmax = 0.0;
-                    for (int r = tid; r < n; r += kT) {
-                        double acc = b[r];
-                        for (int c = 0; c < n; ++c) acc = fma(-A[r + (size_t)c * lda], sm.x[c], acc);
-                        sm.r[r] = acc;
-                        const double ar = fabs(acc), ax = fabs(sm.x[r]);
-                        rmax = (ar > rmax || ar != ar) ? ar : rmax;
-                        xmax = (ax > xmax || ax != ax) ? ax : xmax;
-                    }
-                    const double rn = block_max(rmax, sm.red);
-                    const double xn = block_max(xmax, sm.red);
-                    if (rn <= xn * cte) {
-                        conv = true;
-                        iters = max(iters, it);
-                        break;
-                    }
-                }
-                if (!conv) status = -(MAGMA_MIXED_ITERMAX + 1);
-                for (int i = tid; i < n; i += kT) x[i] = sm.x[i];
-                __syncthreads();
-            }
-        }
-        if (status != 0 && tid == 0) {
-            const int slot = atomicAdd(g.fb_count, 1);
-            g.fb_list[slot] = item;
+        double anrm = 0.0, berr = 0.0;
+#pragma unroll 1
+        for (int g0 = 0; g0 < nrhs && status == 0; g0 += RG) {
+            const int nr = min(RG, nrhs - g0);
+            double be = 0.0;
+            const int r = M::refine(sm, A, lda, g.B[id] + (size_t)g0 * ldb, ldb, g.X[id] + (size_t)g0 * ldx, ldx,
+                                    F, n, nr, chol, anrm, be);
+            if (r < 0) status = r;
+            else iters = max(iters, r);
+            berr = dmax_nan(berr, be);
         }
         if (tid == 0) {
-            g.iter[item] = status == 0 ? iters : status;
-            g.info[item] = 0;
+            if (status != 0) {
+                if (chol) g.fbc_list[atomicAdd(g.fbc_count, 1)] = id;
+                else g.fb_list[atomicAdd(g.fb_count, 1)] = id;
+            }
+            if (g.iter) g.iter[id] = status == 0 ? iters : status;
+            if (g.info) g.info[id] = 0;
+            if (g.berr) g.berr[id] = berr;
         }
         __syncthreads();
     }
 }
 
-// X <- B for the matrices that fall back (the FP64 solve overwrites X in place)
-__global__ void copy_rhs_kernel(const int *list, const int *count, double **B, int ldb, double **X, int ldx,
-                                int n, int nrhs)
+// X <- B for the listed matrices (the FP64 solve then works in place on X)
+__global__ void copy_rhs_kernel(MixedSet g, const int *list, const int *count)
 {
     const int cnt = *count;
     for (int t = blockIdx.x; t < cnt; t += gridDim.x) {
         const int id = list[t];
+        const int n = pick(g.n_arr, g.n, id), nrhs = pick(g.nrhs_arr, g.nrhs, id);
+        const int ldb = pick(g.ldb_arr, g.ldb, id), ldx = pick(g.ldx_arr, g.ldx, id);
         for (int e = threadIdx.x; e < n * nrhs; e += blockDim.x) {
             const int i = e % n, c = e / n;
-            X[id][i + (size_t)c * ldx] = B[id][i + (size_t)c * ldb];
+            g.X[id][i + (size_t)c * ldx] = g.B[id][i + (size_t)c * ldb];
         }
     }
 }
 
-__global__ void zero_one_kernel(int *p) { *p = 0; }
+__global__ void zero_counts_kernel(int *a, int *b)
+{
+    if (threadIdx.x == 0) {
+        if (a) *a = 0;
+        if (b) *b = 0;
+    }
+}
 
 __global__ void fill_fallback_all_kernel(int *list, int *count, int *iter, int batch)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < batch) {
         list[i] = i;
-        iter[i] = -1;
+        if (iter) iter[i] = -1;
     }
     if (i == 0) *count = batch;
 }
 
 }  // namespace
+
+// Workgroups resident per CU for a kernel at 256 threads (LDS and registers).
+template <class K>
+static int resident_per_cu(K kernel)
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, kT, 0) != hipSuccess || nb < 1) nb = 1;
+    return nb;
+}
+
+// Kernel variants (block-column width NB, right-hand-side group RG):
+//   n <= 256: 0 (NB 128, RG 16), 1 (NB 64, RG 8), 2 (NB 32, RG 8)
+//   n <= 512: 0 (NB 64, RG 8), 1 (NB 32, RG 8)
+static int mixed_variant(int max_n)
+{
+    return tuned_variant(kTuneMixed, size_class_of(max_n <= 256 ? 256 : 512));
+}
+
+int mixed_grid(int count, int max_n)
+{
+    const int v = mixed_variant(max_n);
+    int per_cu;
+    if (max_n <= 256) {
+        per_cu = v == 0 ? resident_per_cu(mixed_kernel<256, 128, 16>)
+               : v == 1 ? resident_per_cu(mixed_kernel<256, 64, 8>) : resident_per_cu(mixed_kernel<256, 32, 8>);
+    } else {
+        per_cu = v == 0 ? resident_per_cu(mixed_kernel<512, 64, 8>) : resident_per_cu(mixed_kernel<512, 32, 8>);
+    }
+    return persistent_grid(count, per_cu);
+}
+
+magma_int_t launch_mixed(const MixedSet &g, int max_n, hipStream_t st)
+{
+    if (g.count <= 0) return MAGMA_SUCCESS;
+    if (max_n > kMixedMaxN) return MAGMA_ERR_NOT_SUPPORTED;
+    const int grid = mixed_grid(g.count, max_n);
+    const int v = mixed_variant(max_n);
+    if (max_n <= 256) {
+        if (v == 0) mixed_kernel<256, 128, 16><<<grid, kT, 0, st>>>(g);
+        else if (v == 1) mixed_kernel<256, 64, 8><<<grid, kT, 0, st>>>(g);
+        else mixed_kernel<256, 32, 8><<<grid, kT, 0, st>>>(g);
+    } else {
+        if (v == 0) mixed_kernel<512, 64, 8><<<grid, kT, 0, st>>>(g);
+        else mixed_kernel<512, 32, 8><<<grid, kT, 0, st>>>(g);
+    }
+    MAGMA_LAUNCH_CHECK();
+    return MAGMA_SUCCESS;
+}
+
+magma_int_t launch_mixed_fallback(const MixedSet &g, int max_n, int max_nrhs, hipStream_t st)
+{
+    (void)max_nrhs;
+    const bool var = g.n_arr != nullptr;
+    for (int pass = 0; pass < 2; ++pass) {
+        const bool chol = pass == 1;
+        int *list = chol ? g.fbc_list : g.fb_list;
+        int *count = chol ? g.fbc_count : g.fb_count;
+        if (!list || !count) continue;
+        copy_rhs_kernel<<<persistent_grid(g.count, 4), 256, 0, st>>>(g, list, count);
+        MAGMA_LAUNCH_CHECK();
+        MatSet s;
+        s.m = g.n; s.n = g.n; s.lda = g.lda;
+        s.m_arr = g.n_arr; s.n_arr = g.n_arr; s.lda_arr = g.lda_arr;
+        s.A = g.A; s.ipiv = g.ipiv; s.info = g.info;
+        s.B = g.X; s.nrhs = g.nrhs; s.ldb = g.ldx;
+        s.nrhs_arr = g.nrhs_arr; s.ldb_arr = g.ldx_arr;
+        s.index = list; s.dev_count = count; s.count = g.count;
+        magma_int_t r;
+        if (!chol) {
+            if (max_n <= 64) {
+                r = launch_getrf_small_v(s, max_n, true, var, st);
+            } else {
+                r = (max_n <= 512) ? launch_getrf_mfma(s, max_n, st) : launch_getrf_blocked(s, max_n, st);
+                if (r == MAGMA_SUCCESS) r = launch_getrs(s, 'N', max_n, 0, st);
+            }
+        } else {
+            if (max_n <= 64) {
+                r = launch_potrf_small_v(s, max_n, true, var, st);
+            } else {
+                r = launch_potrf_blocked(s, max_n, st);
+                if (r == MAGMA_SUCCESS) r = launch_potrs(s, 'L', max_n, 0, st);
+            }
+        }
+        if (r != MAGMA_SUCCESS) return r;
+    }
+    return MAGMA_SUCCESS;
+}
+
 }  // namespace magma_amd
 
 using namespace magma_amd;
+
+namespace {
+
+// shared driver of the fixed-size entry points
+magma_int_t mixed_batched(char kind, magma_int_t n, magma_int_t nrhs, double **dA_array, magma_int_t ldda,
+                          magma_int_t **dipiv_array, double **dB_array, magma_int_t lddb, double **dX_array,
+                          magma_int_t lddx, magma_int_t *dinfo_array, magma_int_t *diter_array,
+                          magma_int_t batchCount, magma_queue_t queue)
+{
+    hipStream_t st = stream_of(queue);
+    const bool mixed_ok = n <= kMixedMaxN;
+    const int grid = mixed_grid(batchCount, n);
+    const size_t slot = mixed_factor_floats(n);
+    const size_t wbytes = mixed_ok ? round_up(sizeof(float) * slot * grid, 256) : 0;
+    const size_t lbytes = sizeof(int) * (size_t)(batchCount + 2);
+    char *ws = static_cast<char *>(queue_workspace(queue, wbytes + lbytes));
+    if (!ws) return MAGMA_ERR_DEVICE_ALLOC;
+    int *fb_count = reinterpret_cast<int *>(ws + wbytes);
+    int *fb_list = fb_count + 1;
+    zero_counts_kernel<<<1, 64, 0, st>>>(fb_count, nullptr);
+    MAGMA_LAUNCH_CHECK();
+    MixedSet g;
+    g.n = n; g.nrhs = nrhs; g.lda = ldda; g.ldb = lddb; g.ldx = lddx;
+    g.kind = kind;
+    g.A = dA_array; g.B = dB_array; g.X = dX_array;
+    g.Wslot = reinterpret_cast<float *>(ws);
+    g.slot_floats = slot;
+    g.ipiv = dipiv_array; g.info = dinfo_array; g.iter = diter_array;
+    if (kind == 'C') {
+        g.fbc_list = fb_list; g.fbc_count = fb_count;
+    } else {
+        g.fb_list = fb_list; g.fb_count = fb_count;
+    }
+    g.count = batchCount;
+    if (mixed_ok) {
+        magma_int_t r = launch_mixed(g, n, st);
+        if (r != MAGMA_SUCCESS) return r;
+    } else {
+        // n above the mixed kernels' range: every matrix takes the FP64 path
+        // (dsgesv's iter = -1, "implementation-specific reasons")
+        fill_fallback_all_kernel<<<ceil_div(batchCount, 256), 256, 0, st>>>(fb_list, fb_count, diter_array,
+                                                                             batchCount);
+        MAGMA_LAUNCH_CHECK();
+    }
+    magma_int_t r = launch_mixed_fallback(g, n, nrhs, st);
+    if (r != MAGMA_SUCCESS) return r;
+    if (!queue->async) MAGMA_HIP_CHECK(hipStreamSynchronize(st));
+    return MAGMA_SUCCESS;
+}
+
+}  // namespace
 
 extern "C" magma_int_t magma_dsgesv_batched(magma_int_t n, magma_int_t nrhs,
                                             double **dA_array, magma_int_t ldda, magma_int_t **dipiv_array,
@@ -536,44 +1059,36 @@ extern "C" magma_int_t magma_dsgesv_batched(magma_int_t n, magma_int_t nrhs,
         return info;
     }
     if (n == 0 || nrhs == 0 || batchCount == 0) return MAGMA_SUCCESS;
-    hipStream_t st = stream_of(queue);
-    const int grid = persistent_grid(batchCount, 4);
-    const bool mixed_ok = n <= kMaxN;
-    const size_t wbytes = mixed_ok ? sizeof(float) * (size_t)n * n * grid : 0;
-    const size_t lbytes = sizeof(int) * (size_t)(batchCount + 1);
-    char *ws = static_cast<char *>(queue_workspace(queue, round_up(wbytes, 256) + lbytes));
-    if (!ws) return MAGMA_ERR_DEVICE_ALLOC;
-    int *fb_count = reinterpret_cast<int *>(ws + round_up(wbytes, 256));
-    int *fb_list = fb_count + 1;
-    zero_one_kernel<<<1, 1, 0, st>>>(fb_count);
-    MixedArgs g;
-    g.n = n; g.nrhs = nrhs; g.lda = ldda; g.ldb = lddb; g.ldx = lddx;
-    g.A = dA_array; g.B = dB_array; g.X = dX_array;
-    g.ipiv = dipiv_array; g.info = dinfo_array; g.iter = diter_array;
-    g.W = reinterpret_cast<float *>(ws);
-    g.fb_list = fb_list; g.fb_count = fb_count;
-    g.count = batchCount;
-    if (mixed_ok) {
-        if (n <= kT) dsgesv_kernel<1><<<grid, kT, 0, st>>>(g);
-        else dsgesv_kernel<2><<<grid, kT, 0, st>>>(g);
-        MAGMA_LAUNCH_CHECK();
-    } else {
-        // n above the mixed kernel's range: every matrix takes the FP64 path
-        // (dsgesv's iter = -1, "implementation-specific reasons")
-        fill_fallback_all_kernel<<<ceil_div(batchCount, 256), 256, 0, st>>>(fb_list, fb_count, diter_array,
-                                                                             batchCount);
-        MAGMA_LAUNCH_CHECK();
+    return mixed_batched('L', n, nrhs, dA_array, ldda, dipiv_array, dB_array, lddb, dX_array, lddx, dinfo_array,
+                         diter_array, batchCount, queue);
+}
+
+extern "C" magma_int_t magma_dsposv_batched(magma_uplo_t uplo, magma_int_t n, magma_int_t nrhs,
+                                            double **dA_array, magma_int_t ldda,
+                                            double **dB_array, magma_int_t lddb,
+                                            double **dX_array, magma_int_t lddx,
+                                            magma_int_t *dinfo_array, magma_int_t *diter_array,
+                                            magma_int_t batchCount, magma_queue_t queue)
+{
+    // argument checks and codes of magma_dsposv_gpu (src/zcposv_gpu.cpp:163-174),
+    // which numbers n, nrhs and ldda as if uplo were not an argument
+    magma_int_t info = 0;
+    if (uplo != MagmaLower && uplo != MagmaUpper) info = -1;
+    else if (n < 0) info = -1;
+    else if (nrhs < 0) info = -2;
+    else if (ldda < std::max(1, n)) info = -4;
+    else if (lddb < std::max(1, n)) info = -7;
+    else if (lddx < std::max(1, n)) info = -9;
+    else if (batchCount < 0) info = -12;
+    if (info == 0 && uplo == MagmaUpper) {
+        std::fprintf(stderr, "%s: uplo=upper is not implemented\n", __func__);
+        info = -1;
     }
-    // FP64 fallback on the device list: X <- B, getrf(A), getrs(X)
-    copy_rhs_kernel<<<persistent_grid(batchCount, 4), 256, 0, st>>>(fb_list, fb_count, dB_array, lddb, dX_array,
-                                                                    lddx, n, nrhs);
-    MatSet s;
-    s.m = n; s.n = n; s.lda = ldda;
-    s.A = dA_array; s.ipiv = dipiv_array; s.info = dinfo_array;
-    s.B = dX_array; s.nrhs = nrhs; s.ldb = lddx;
-    s.index = fb_list; s.dev_count = fb_count; s.count = batchCount;
-    magma_int_t r = launch_getrf(s, n, n, true, st);
-    if (r != MAGMA_SUCCESS) return r;
-    if (!queue->async) MAGMA_HIP_CHECK(hipStreamSynchronize(st));
-    return MAGMA_SUCCESS;
+    if (info != 0) {
+        magma_xerbla(__func__, -info);
+        return info;
+    }
+    if (n == 0 || nrhs == 0 || batchCount == 0) return MAGMA_SUCCESS;
+    return mixed_batched('C', n, nrhs, dA_array, ldda, nullptr, dB_array, lddb, dX_array, lddx, dinfo_array,
+                         diter_array, batchCount, queue);
 }

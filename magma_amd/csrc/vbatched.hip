@@ -17,14 +17,16 @@
 
 #include "kernels.h"
 #include "workqueue.h"
+#include "tune.h"
 
 namespace magma_amd {
 
 magma_int_t launch_getrf_small_v(const MatSet &s, int max_mn_dim, bool solve, bool var, hipStream_t st);
 magma_int_t launch_potrf_small_v(const MatSet &s, int max_n, bool solve, bool var, hipStream_t st);
 magma_int_t launch_getrf_blocked(const MatSet &s, int max_m, hipStream_t st);
+magma_int_t launch_getrf_mfma(const MatSet &s, int max_dim, hipStream_t st);
 magma_int_t launch_potrf_blocked(const MatSet &s, int max_n, hipStream_t st);
-magma_int_t launch_gemm_tiles(const GemmSet &g, const int *prefix, long max_tiles, hipStream_t st);
+magma_int_t launch_gemm_tiles(const GemmSet &g, const int *prefix, long max_tiles, int tile, hipStream_t st);
 magma_int_t zero_info(int *info, int count, hipStream_t st);
 
 namespace {
@@ -136,7 +138,9 @@ magma_int_t launch_class(char kind, int b, MatSet s, hipStream_t st)
 {
     const int lim = kBucketLimit[b];
     switch (kind) {
-    case 'L': return lim <= 64 ? launch_getrf_small_v(s, lim, false, true, st) : launch_getrf_blocked(s, lim, st);
+    case 'L':
+        if (lim <= 64) return launch_getrf_small_v(s, lim, false, true, st);
+        return lim <= 512 ? launch_getrf_mfma(s, lim, st) : launch_getrf_blocked(s, lim, st);
     case 'C': return lim <= 64 ? launch_potrf_small_v(s, lim, false, true, st) : launch_potrf_blocked(s, lim, st);
     case 'N':
     case 'T': return launch_getrs(s, kind, lim, 0, st);
@@ -274,6 +278,32 @@ magma_int_t magma_dgetrs_vbatched(magma_trans_t trans, magma_int_t *n, magma_int
     return finish(run_classes(trans == MagmaNoTrans ? 'N' : 'T', s, v, stream_of(queue)), queue);
 }
 
+// Unchecked, never synchronizing (capturable in a HIP graph): max_n bounds
+// every n[i]; nrhs[i] and lddb[i] are not validated.
+magma_int_t magma_dgetrs_vbatched_max_nocheck(magma_trans_t trans, magma_int_t *n, magma_int_t *nrhs,
+                                              double **dA_array, magma_int_t *ldda, magma_int_t **dipiv_array,
+                                              double **dB_array, magma_int_t *lddb,
+                                              magma_int_t batchCount, magma_int_t max_n, magma_queue_t queue)
+{
+    if (trans != MagmaNoTrans && trans != MagmaTrans && trans != MagmaConjTrans) {
+        magma_xerbla(__func__, 1);
+        return -1;
+    }
+    if (batchCount <= 0) return MAGMA_SUCCESS;
+    if (max_n > kMaxFactorDim) return MAGMA_ERR_NOT_SUPPORTED;
+    VScratch v;
+    if (!vscratch(queue, batchCount, v)) return MAGMA_ERR_DEVICE_ALLOC;
+    hipStream_t st = stream_of(queue);
+    magma_int_t r = build_workqueue(nullptr, n, batchCount, v.q, st);
+    if (r != MAGMA_SUCCESS) return r;
+    MatSet s;
+    s.n_arr = n; s.m_arr = n; s.lda_arr = ldda;
+    s.nrhs_arr = nrhs; s.ldb_arr = lddb;
+    s.A = dA_array; s.ipiv = dipiv_array; s.B = dB_array;
+    s.count = batchCount;
+    return finish(run_classes_device(trans == MagmaNoTrans ? 'N' : 'T', s, v, max_n, st), queue);
+}
+
 // ---------------------------------------------------------------- Cholesky ---
 // src/zpotrf_vbatched.cpp:335-356; checks magmablas/vbatched_check.cu:840-892
 magma_int_t magma_dpotrf_vbatched(magma_uplo_t uplo, magma_int_t *n, double **dA_array, magma_int_t *ldda,
@@ -368,6 +398,31 @@ magma_int_t magma_dpotrs_vbatched(magma_uplo_t uplo, magma_int_t *n, magma_int_t
     return finish(run_classes(uplo == MagmaLower ? 'l' : 'u', s, v, stream_of(queue)), queue);
 }
 
+// Unchecked, never synchronizing (capturable in a HIP graph).
+magma_int_t magma_dpotrs_vbatched_max_nocheck(magma_uplo_t uplo, magma_int_t *n, magma_int_t *nrhs,
+                                              double **dA_array, magma_int_t *ldda, double **dB_array,
+                                              magma_int_t *lddb, magma_int_t batchCount, magma_int_t max_n,
+                                              magma_queue_t queue)
+{
+    if (uplo != MagmaLower && uplo != MagmaUpper) {
+        magma_xerbla(__func__, 1);
+        return -1;
+    }
+    if (batchCount <= 0) return MAGMA_SUCCESS;
+    if (max_n > kMaxFactorDim) return MAGMA_ERR_NOT_SUPPORTED;
+    VScratch v;
+    if (!vscratch(queue, batchCount, v)) return MAGMA_ERR_DEVICE_ALLOC;
+    hipStream_t st = stream_of(queue);
+    magma_int_t r = build_workqueue(nullptr, n, batchCount, v.q, st);
+    if (r != MAGMA_SUCCESS) return r;
+    MatSet s;
+    s.n_arr = n; s.m_arr = n; s.lda_arr = ldda;
+    s.nrhs_arr = nrhs; s.ldb_arr = lddb;
+    s.A = dA_array; s.B = dB_array;
+    s.count = batchCount;
+    return finish(run_classes_device(uplo == MagmaLower ? 'l' : 'u', s, v, max_n, st), queue);
+}
+
 // ---------------------------------------------------------------- GEMM -------
 static char tr(magma_trans_t t) { return t == MagmaNoTrans ? 'N' : 'T'; }
 
@@ -399,8 +454,9 @@ void magmablas_dgemm_batched(magma_trans_t transA, magma_trans_t transB,
     g.alpha = alpha; g.beta = beta;
     g.A = dA_array; g.B = dB_array; g.C = dC_array;
     g.count = batchCount;
-    const long tiles = (long)ceil_div(m, 32) * ceil_div(n, 32) * batchCount;
-    launch_gemm_tiles(g, nullptr, tiles, stream_of(queue));
+    const int tile = tuned_variant(kTuneGemm, size_class_of(std::max(m, n))) == 1 ? 16 : 32;
+    const long tiles = (long)ceil_div(m, tile) * ceil_div(n, tile) * batchCount;
+    launch_gemm_tiles(g, nullptr, tiles, tile, stream_of(queue));
 }
 
 // magmablas/zgemm_batched.cpp:166-195 (no vendor-BLAS branch: one implementation)
@@ -430,15 +486,19 @@ void magmablas_dgemm_vbatched_max_nocheck(magma_trans_t transA, magma_trans_t tr
         return;
     }
     hipStream_t st = stream_of(queue);
-    if (build_tile_prefix(m, n, batchCount, 32, 32, v.prefix, st) != MAGMA_SUCCESS) return;
+    // one tile size for the whole call: the class of the largest matrix, or the
+    // mixed-size class when the caller's bounds are not tight
+    const int cls = (max_m > 2048 || max_n > 2048) ? kTuneClasses - 1 : size_class_of(std::max(max_m, max_n));
+    const int tile = tuned_variant(kTuneGemm, cls) == 1 ? 16 : 32;
+    if (build_tile_prefix(m, n, batchCount, tile, tile, v.prefix, st) != MAGMA_SUCCESS) return;
     GemmSet g;
     g.ta = tr(transA); g.tb = tr(transB);
     g.m_arr = m; g.n_arr = n; g.k_arr = k; g.lda_arr = ldda; g.ldb_arr = lddb; g.ldc_arr = lddc;
     g.alpha = alpha; g.beta = beta;
     g.A = dA_array; g.B = dB_array; g.C = dC_array;
     g.count = batchCount;
-    const long tiles = (long)ceil_div(max_m, 32) * ceil_div(max_n, 32) * batchCount;
-    launch_gemm_tiles(g, v.prefix, tiles, st);
+    const long tiles = (long)ceil_div(std::min(max_m, 4096), tile) * ceil_div(std::min(max_n, 4096), tile) * batchCount;
+    launch_gemm_tiles(g, v.prefix, tiles, tile, st);
 }
 
 // magmablas/zgemm_vbatched.cpp:235-270

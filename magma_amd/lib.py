@@ -27,6 +27,31 @@ MAGMA_ERR_DEVICE_ALLOC = -113
 MAGMA_ERR_NOT_SUPPORTED = -103
 MAGMA_ERR_NOT_IMPLEMENTED = -117
 MAGMA_MIXED_ITERMAX = 10
+MAGMA_AMD_SOLVE_LU, MAGMA_AMD_SOLVE_SPD = 0, 1
+MAGMA_AMD_TUNE_GETRF, MAGMA_AMD_TUNE_MIXED, MAGMA_AMD_TUNE_GEMM, MAGMA_AMD_TUNE_VERBOSE = 1, 2, 4, 256
+
+
+class Request(ctypes.Structure):
+    """magma_amd_request_t (include/magma_amd.h)"""
+    _fields_ = [("kind", c_int), ("n", c_int), ("nrhs", c_int),
+                ("dA", ctypes.c_void_p), ("ldda", c_int),
+                ("dB", ctypes.c_void_p), ("lddb", c_int),
+                ("dX", ctypes.c_void_p), ("lddx", c_int),
+                ("factor_key", ctypes.c_longlong)]
+
+
+class Completion(ctypes.Structure):
+    """magma_amd_completion_t"""
+    _fields_ = [("ticket", ctypes.c_longlong), ("info", c_int), ("iter", c_int), ("reused", c_int),
+                ("berr", c_double), ("t_submit", c_double), ("t_start", c_double), ("t_done", c_double)]
+
+
+class ServerStats(ctypes.Structure):
+    """magma_amd_server_stats_t"""
+    _fields_ = [("submitted", ctypes.c_longlong), ("completed", ctypes.c_longlong),
+                ("batches", ctypes.c_longlong), ("evictions", ctypes.c_longlong),
+                ("budget_bytes", ctypes.c_size_t), ("bytes_in_use", ctypes.c_size_t),
+                ("bytes_peak", ctypes.c_size_t), ("resident", c_int), ("max_batch_seen", c_int)]
 
 # name -> (restype, argtypes); every pointer argument is passed as an integer address
 _P = c_void_p
@@ -80,6 +105,7 @@ PROTOTYPES = {
                                        c_double, _P, c_int, c_int, _P]),
     # mixed precision (include/magma_amd.h)
     "magma_dsgesv_batched": (c_int, [c_int, c_int, _P, c_int, _P, _P, c_int, _P, c_int, _P, _P, c_int, _P]),
+    "magma_dsposv_batched": (c_int, [c_int, c_int, c_int, _P, c_int, _P, c_int, _P, c_int, _P, _P, c_int, _P]),
     # vbatched (include/magma_dvbatched.h)
     "magma_dgetrf_vbatched": (c_int, [_P, _P, _P, _P, _P, _P, c_int, _P]),
     "magma_dgetrf_vbatched_max_nocheck": (c_int, [_P, _P, c_int, _P, _P, _P, _P, c_int, _P]),
@@ -87,6 +113,8 @@ PROTOTYPES = {
     "magma_dpotrf_vbatched": (c_int, [c_int, _P, _P, _P, _P, c_int, _P]),
     "magma_dpotrf_vbatched_max_nocheck": (c_int, [c_int, _P, _P, _P, _P, c_int, c_int, _P]),
     "magma_dpotrs_vbatched": (c_int, [c_int, _P, _P, _P, _P, _P, _P, c_int, _P]),
+    "magma_dgetrs_vbatched_max_nocheck": (c_int, [c_int, _P, _P, _P, _P, _P, _P, _P, c_int, c_int, _P]),
+    "magma_dpotrs_vbatched_max_nocheck": (c_int, [c_int, _P, _P, _P, _P, _P, _P, c_int, c_int, _P]),
     "magmablas_dgemm_vbatched": (None, [c_int, c_int, _P, _P, _P, c_double, _P, _P, _P, _P, c_double,
                                         _P, _P, c_int, _P]),
     "magmablas_dgemm_vbatched_max_nocheck": (None, [c_int, c_int, _P, _P, _P, c_double, _P, _P, _P, _P,
@@ -95,6 +123,19 @@ PROTOTYPES = {
                                             _P, _P, c_int, c_int, c_int, c_int, _P]),
     "magmablas_dgemm_vbatched_nocheck": (None, [c_int, c_int, _P, _P, _P, c_double, _P, _P, _P, _P,
                                                 c_double, _P, _P, c_int, _P]),
+    "magma_amd_autotune": (c_int, [_P, c_int]),
+    "magma_amd_tune_get": (c_int, [c_int, c_int]),
+    "magma_amd_tune_set": (c_int, [c_int, c_int, c_int]),
+    "magma_amd_tune_save": (c_int, [c_char_p]),
+    "magma_amd_tune_load": (c_int, [c_char_p]),
+    "magma_amd_server_create": (c_int, [c_int, ctypes.c_size_t, c_int, _P]),
+    "magma_amd_server_submit": (ctypes.c_longlong, [_P, _P]),
+    "magma_amd_server_poll": (c_int, [_P, _P, c_int]),
+    "magma_amd_server_wait": (c_int, [_P, _P, c_int, c_double]),
+    "magma_amd_server_drain": (c_int, [_P]),
+    "magma_amd_server_stats": (c_int, [_P, _P]),
+    "magma_amd_server_run_trace": (c_int, [_P, _P, _P, c_int, _P]),
+    "magma_amd_server_destroy": (None, [_P]),
 }
 
 OPTIONAL = set()

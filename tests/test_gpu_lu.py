@@ -12,7 +12,8 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-SIZES = [1, 2, 3, 5, 8, 9, 13, 16, 17, 24, 31, 32, 33, 40, 48, 63, 64, 65, 96, 100, 128, 129, 200, 256]
+SIZES = [1, 2, 3, 5, 8, 9, 13, 16, 17, 24, 31, 32, 33, 40, 48, 63, 64, 65, 96, 100, 128, 129, 200, 256, 257, 300,
+         384, 512, 513, 600]
 
 
 def _batch(count, n, seed, m=None, ld=None):
@@ -50,7 +51,8 @@ def test_getrf_bitexact(queue, oracle, n):
         assert oracle.lu_error(A[s], LU[s], ipiv[s]) < 30
 
 
-@pytest.mark.parametrize("m,n", [(20, 12), (12, 20), (64, 40), (40, 64), (150, 90), (90, 150)])
+@pytest.mark.parametrize("m,n", [(20, 12), (12, 20), (64, 40), (40, 64), (150, 90), (90, 150), (10, 250),
+                                 (250, 10), (400, 260), (260, 400), (512, 33), (33, 512)])
 def test_getrf_rectangular(queue, oracle, m, n):
     count = 50
     A = _batch(count, n, seed=7, m=m)
@@ -62,7 +64,7 @@ def test_getrf_rectangular(queue, oracle, m, n):
         assert same(LU[s], LUo)
 
 
-@pytest.mark.parametrize("n,ld", [(32, 40), (64, 70), (100, 128)])
+@pytest.mark.parametrize("n,ld", [(32, 40), (64, 70), (100, 128), (200, 256), (300, 301)])
 def test_getrf_padded_ld(queue, oracle, n, ld):
     A = _batch(20, n, seed=3)
     LU, ipiv, info = run_getrf(queue, A, n, n, ld=ld)
@@ -70,7 +72,7 @@ def test_getrf_padded_ld(queue, oracle, n, ld):
     assert np.array_equal(ipiv, ipivo) and same(LU, LUo)
 
 
-@pytest.mark.parametrize("n", [6, 32, 50, 130])
+@pytest.mark.parametrize("n", [6, 32, 50, 130, 300])
 def test_getrf_singular_ties_nan(queue, oracle, n):
     rng = np.random.default_rng(n)
     mats = []

@@ -1,11 +1,12 @@
-"""GPU: magma_dsgesv_batched (FP32 factorization with split-FP16 MFMA updates,
-FP64 refinement, FP64 fallback).
+"""GPU: magma_dsgesv_batched / magma_dsposv_batched (FP32 factorization with
+split-FP16 MFMA updates, FP64 refinement, FP64 fallback).
 
 Bar: every solution has backward error ||b - Ax||_inf / (||A||_inf ||x||_inf)
 <= n * eps64 (oracle-computed, orc_solve_error <= 1); matrices within the
 refinement's reach (cond <= 1e6) converge without fallback and leave A and B
 unchanged; singular, NaN and very ill-conditioned matrices take the FP64 path
-with dsgesv's iter codes and then meet the plain solver's bar (< 30).
+with dsgesv's iter codes and then meet the plain solver's bar (< 30), with A
+holding the FP64 factors bit-exact as the plain path produces them.
 """
 import numpy as np
 import pytest
@@ -13,7 +14,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def run_dsgesv(queue, A, B):
+def run_mixed(queue, A, B, kind="L"):
     import magma_amd as ma
     import torch
     count, n, _ = A.shape
@@ -24,27 +25,69 @@ def run_dsgesv(queue, A, B):
     ipiv = ma.IntBatch(count, n)
     info = torch.zeros(count, dtype=torch.int32, device="cuda")
     it = torch.full((count,), 99, dtype=torch.int32, device="cuda")
-    print(f"dsgesv count={count} n={n} nrhs={nrhs}", flush=True)
-    ret = ma.load().magma_dsgesv_batched(n, nrhs, dA.ptr, dA.ld, ipiv.ptr, dB.ptr, dB.ld, dX.ptr, dX.ld,
-                                         info.data_ptr(), it.data_ptr(), count, queue.handle)
+    L = ma.load()
+    if kind == "L":
+        print(f"dsgesv count={count} n={n} nrhs={nrhs}", flush=True)
+        ret = L.magma_dsgesv_batched(n, nrhs, dA.ptr, dA.ld, ipiv.ptr, dB.ptr, dB.ld, dX.ptr, dX.ld,
+                                     info.data_ptr(), it.data_ptr(), count, queue.handle)
+    else:
+        print(f"dsposv count={count} n={n} nrhs={nrhs}", flush=True)
+        ret = L.magma_dsposv_batched(ma.MagmaLower, n, nrhs, dA.ptr, dA.ld, dB.ptr, dB.ld, dX.ptr, dX.ld,
+                                     info.data_ptr(), it.data_ptr(), count, queue.handle)
     assert ret == 0
     return dA.numpy(), dB.numpy(), dX.numpy(), ipiv.numpy(), info.cpu().numpy(), it.cpu().numpy()
 
 
+def spd_cond_batch(seed, count, n, kmax=1e6):
+    """SPD matrices U diag(s) U' with cond log-uniform in [1e2, kmax]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        U, _ = np.linalg.qr(rng.standard_normal((n, n)))
+        kappa = 10.0 ** rng.uniform(2.0, np.log10(kmax))
+        s = kappa ** (-np.arange(n) / max(n - 1, 1))
+        S = (U * s) @ U.T
+        out.append(np.ascontiguousarray((S + S.T) / 2))
+    return np.stack(out)
+
+
 @pytest.mark.parametrize("n,nrhs,count", [(256, 1, 48), (100, 2, 40), (33, 1, 40), (300, 1, 12), (1, 1, 4),
-                                          (64, 5, 20)])
+                                          (64, 5, 20), (512, 2, 6), (200, 16, 8), (130, 21, 6), (17, 3, 30)])
 def test_dsgesv_cond_matrices(queue, oracle, n, nrhs, count):
     from magma_amd import gen
     A, kappa = gen.cond_batch(3, count, n) if n > 1 else (np.full((count, 1, 1), 0.5), np.ones(count))
     B = gen.rhs_batch(3, count, n, nrhs)
-    Ag, Bg, X, ipiv, info, it = run_dsgesv(queue, A, B)
+    Ag, Bg, X, ipiv, info, it = run_mixed(queue, A, B)
     assert np.all(info == 0)
     assert np.all(it >= 0), f"fallbacks: {np.sum(it < 0)} iter={it}"
     assert np.all(it <= 10)
     assert np.array_equal(Ag, A) and np.array_equal(Bg, B)       # untouched when refinement succeeds
+    # the FP32 factorization's pivots are LAPACK-form: ipiv[i] in [i+1, n]
+    idx = np.arange(1, n + 1)
+    assert np.all((ipiv >= idx) & (ipiv <= n))
     for s in range(count):
         be = oracle.solve_error(A[s], X[s], B[s])                 # ||r|| / (n ||A|| ||x|| eps)
         assert be <= 1.0, (s, be, kappa[s], it[s])
+
+
+def test_dsgesv_pivots_match_lapack_sgetrf_on_separated_columns(queue, oracle):
+    """With one dominant entry per column the pivot choice has no near ties,
+    so the FP32 factorization's interchanges are LAPACK's."""
+    rng = np.random.default_rng(5)
+    n, count = 96, 10
+    mats = []
+    for _ in range(count):
+        perm = rng.permutation(n)
+        M = rng.uniform(-1, 1, (n, n)) * 1e-3
+        M[perm, np.arange(n)] = rng.uniform(1, 2, n) * rng.choice([-1, 1], n)
+        mats.append(np.ascontiguousarray(M.T))
+    A = np.stack(mats)
+    B = np.ones((count, 1, n))
+    Ag, Bg, X, ipiv, info, it = run_mixed(queue, A, B)
+    assert np.all(it >= 0)
+    for s in range(count):
+        _, ipo, _ = oracle.dgetrf(A[s])
+        assert np.array_equal(ipiv[s], ipo), s
 
 
 def test_dsgesv_fallbacks(queue, oracle):
@@ -61,7 +104,7 @@ def test_dsgesv_fallbacks(queue, oracle):
     s = 10.0 ** (-13 * np.arange(n) / (n - 1))
     A[4] = np.ascontiguousarray(((U * s) @ V.T).T)
     B = gen.rhs_batch(5, count, n, 1)
-    Ag, Bg, X, ipiv, info, it = run_dsgesv(queue, A, B)
+    Ag, Bg, X, ipiv, info, it = run_mixed(queue, A, B)
     assert it[0] >= 0 and it[5] >= 0
     assert it[1] == -3 and info[1] == 1
     assert it[2] < 0                                              # singular: FP32 pivot may be tiny, not zero
@@ -80,10 +123,52 @@ def test_dsgesv_large_n_falls_back(queue, oracle):
     n, count = 600, 2
     A = gen.general_batch(8, count, n)
     B = gen.rhs_batch(8, count, n, 1)
-    Ag, Bg, X, ipiv, info, it = run_dsgesv(queue, A, B)
+    Ag, Bg, X, ipiv, info, it = run_mixed(queue, A, B)
     assert np.all(it == -1)
     for s in range(count):
         assert oracle.solve_error(A[s], X[s], B[s]) < 30
+
+
+@pytest.mark.parametrize("n,nrhs,count", [(1, 1, 4), (20, 3, 30), (64, 4, 20), (100, 1, 20), (256, 2, 12),
+                                          (300, 5, 6), (512, 1, 4)])
+def test_dsposv_spd(queue, oracle, n, nrhs, count):
+    from magma_amd import gen
+    A = gen.spd_batch(4, count, n)
+    B = gen.rhs_batch(4, count, n, nrhs)
+    Ag, Bg, X, ipiv, info, it = run_mixed(queue, A, B, kind="C")
+    assert np.all(info == 0)
+    assert np.all((it >= 0) & (it <= 10)), it
+    assert np.array_equal(Ag, A) and np.array_equal(Bg, B)
+    for s in range(count):
+        assert oracle.solve_error(A[s], X[s], B[s]) <= 1.0, s
+
+
+@pytest.mark.parametrize("n", [48, 200])
+def test_dsposv_conditioned(queue, oracle, n):
+    from magma_amd import gen
+    A = spd_cond_batch(9, 10, n)
+    B = gen.rhs_batch(9, 10, n, 2)
+    Ag, Bg, X, ipiv, info, it = run_mixed(queue, A, B, kind="C")
+    assert np.all(info == 0)
+    assert np.all(it >= 0), it
+    for s in range(10):
+        assert oracle.solve_error(A[s], X[s], B[s]) <= 1.0, s
+
+
+def test_dsposv_not_spd_falls_back(queue, oracle):
+    from magma_amd import gen
+    n, count = 70, 4
+    A = gen.spd_batch(6, count, n).copy()
+    A[1][10, 10] = -5.0                   # negative diagonal: not SPD before any update
+    A[2] = -A[2]                          # negative definite
+    A[3][20, 30] = A[3][30, 20] = 1e4     # indefinite after elimination
+    B = gen.rhs_batch(6, count, n, 1)
+    Ag, Bg, X, ipiv, info, it = run_mixed(queue, A, B, kind="C")
+    assert it[0] >= 0 and info[0] == 0
+    for s in (1, 2, 3):
+        assert it[s] < 0, (s, it[s])
+        Lo, info_o = oracle.dpotrf(A[s])
+        assert info[s] == info_o and info_o > 0, (s, info[s], info_o)
 
 
 def test_dsgesv_argument_errors(queue):
@@ -95,3 +180,9 @@ def test_dsgesv_argument_errors(queue):
     assert L.magma_dsgesv_batched(4, 1, 0, 3, 0, 0, 4, 0, 4, 0, 0, 1, q) == -4
     assert L.magma_dsgesv_batched(4, 1, 0, 4, 0, 0, 3, 0, 4, 0, 0, 1, q) == -7
     assert L.magma_dsgesv_batched(4, 1, 0, 4, 0, 0, 4, 0, 3, 0, 0, 1, q) == -9
+    # dsposv keeps magma_dsposv_gpu's codes (n, nrhs, ldda numbered without uplo)
+    assert L.magma_dsposv_batched(ma.MagmaUpper, 4, 1, 0, 4, 0, 4, 0, 4, 0, 0, 1, q) == -1
+    assert L.magma_dsposv_batched(ma.MagmaLower, -1, 1, 0, 4, 0, 4, 0, 4, 0, 0, 1, q) == -1
+    assert L.magma_dsposv_batched(ma.MagmaLower, 4, -1, 0, 4, 0, 4, 0, 4, 0, 0, 1, q) == -2
+    assert L.magma_dsposv_batched(ma.MagmaLower, 4, 1, 0, 3, 0, 4, 0, 4, 0, 0, 1, q) == -4
+    assert L.magma_dsposv_batched(ma.MagmaLower, 4, 1, 0, 4, 0, 3, 0, 4, 0, 0, 1, q) == -7

@@ -56,7 +56,7 @@ __device__ __forceinline__ int pick(const int *arr, int fixed, int id) { return 
 // logical position (first maximum).  Keys are never NaN.
 __device__ __forceinline__ void better(double &key, int &pos, double k2, int p2)
 {
-    const bool take = (k2 > key) || (k2 == key && p2 < pos);
+    const bool take = (k2 > key) | ((k2 == key) & (p2 < pos));
     key = take ? k2 : key;
     pos = take ? p2 : pos;
 }
@@ -113,7 +113,7 @@ struct LuSmem {
 };
 
 template <int MT>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) getrf_mfma_kernel(MatSet s)
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(MT <= 256 ? 2 : 1))) getrf_mfma_kernel(MatSet s)
 {
     constexpr int RPT = MT / kThreads;
     __shared__ LuSmem<MT> sm;
@@ -186,22 +186,36 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2
                         sl.pos = wp;
                     }
                     __syncthreads();
-                    double bk = sm.slot[jj & 1][0].key;
-                    int bp = sm.slot[jj & 1][0].pos;
+                    // all four candidates, then the winner's row: two LDS
+                    // round trips, each issued as one batch of reads
+                    double ck[kWaves];
+                    int cp[kWaves];
+#pragma unroll
+                    for (int q = 0; q < kWaves; ++q) {
+                        ck[q] = sm.slot[jj & 1][q].key;
+                        cp[q] = sm.slot[jj & 1][q].pos;
+                    }
+                    asm volatile("" ::: "memory");
+                    double bk = ck[0];
+                    int bp = cp[0];
                     int bw = 0;
 #pragma unroll
                     for (int q = 1; q < kWaves; ++q) {
-                        const double k2 = sm.slot[jj & 1][q].key;
-                        const int p2 = sm.slot[jj & 1][q].pos;
-                        const bool take = (k2 > bk) || (k2 == bk && p2 < bp);
-                        bk = take ? k2 : bk;
-                        bp = take ? p2 : bp;
+                        const bool take = (ck[q] > bk) | ((ck[q] == bk) & (cp[q] < bp));
+                        bk = take ? ck[q] : bk;
+                        bp = take ? cp[q] : bp;
                         bw = take ? q : bw;
                     }
                     const double *prow = sm.slot[jj & 1][bw].row;
+                    // one batch of reads into registers when they fit beside
+                    // the panel (one panel row per thread); else read per use
+                    double u[PW];
+#pragma unroll
+                    for (int c = jj; c < PW; ++c) u[c] = prow[c];
+                    if (RPT == 1) asm volatile("" ::: "memory");
                     const int p = bp;
                     if (tid == 0) sm.swaps[jj] = p;
-                    const double piv = prow[jj];
+                    const double piv = u[jj];
                     if (piv == 0.0 && info == 0) info = j0 + jj + 1;
                     const bool use_rcp = fabs(piv) >= DBL_MIN;
                     const double rc = 1.0 / piv;
@@ -217,8 +231,9 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2
                             bad |= !__builtin_isfinite(l);
 #pragma unroll
                             for (int c = jj + 1; c < PW; ++c) {
-                                const double u = prow[c];
-                                if (u != 0.0) a[i][c] = fma(l, -u, a[i][c]);
+                                // reference dger: a zero pivot-row entry is skipped
+                                const double v = fma(l, -u[c], a[i][c]);
+                                a[i][c] = (u[c] != 0.0) ? v : a[i][c];
                             }
                         }
                     }
@@ -246,17 +261,26 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2
             // ------------------------------------------------ rows ---------
             const int cb = j0 + jb;                 // first column right of the panel
             const int nother = n - jb;
+            // logical top row r comes from panel row st[r]; indices past jb are
+            // clamped to valid rows (their values are discarded), so every
+            // load is unconditional and all of them are in flight together
+#pragma unroll 1
             for (int t = tid; t < nother; t += kThreads) {
                 const int c = (t < j0) ? t : t + jb;
                 double *col = A + (size_t)c * lda + j0;
                 double x[PW], wv[PW];
 #pragma unroll
-                for (int r = 0; r < PW; ++r) x[r] = (r < jb) ? col[sm.src_top[r]] : 0.0;
+                for (int r = 0; r < PW; ++r) x[r] = col[sm.src_top[min(r, jb - 1)]];
 #pragma unroll
-                for (int r = 0; r < PW; ++r) wv[r] = (r < jb && sm.down_dst[r] >= 0) ? col[r] : 0.0;
+                for (int r = 0; r < PW; ++r) wv[r] = col[min(r, jb - 1)];
 #pragma unroll
-                for (int r = 0; r < PW; ++r)
-                    if (r < jb && sm.down_dst[r] >= 0) col[sm.down_dst[r]] = wv[r];
+                for (int r = 0; r < PW; ++r) x[r] = (r < jb) ? x[r] : 0.0;
+                // the top rows that move below the panel
+#pragma unroll
+                for (int r = 0; r < PW; ++r) {
+                    const int d = (r < jb) ? sm.down_dst[r] : -1;
+                    if (d >= 0) col[d] = wv[r];
+                }
                 if (c >= cb) {
                     // rows of L11 at and beyond jb are never read for k < jb
                     // into a stored result: x[k] = 0 there
@@ -291,6 +315,7 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2
             const int nrt = (m - cb + 31) / 32;
             const int nct = (n - cb + 31) / 32;
             if (m > cb && n > cb) {
+#pragma unroll 1
                 for (int t = w; t < nrt * nct; t += kWaves) {
                     const int r0 = cb + 32 * (t % nrt);
                     const int c0 = cb + 32 * (t / nrt);

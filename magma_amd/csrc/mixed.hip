@@ -140,7 +140,7 @@ __device__ __forceinline__ double wave_max(double v)
 // arg-max key order: larger key, ties to the smaller current position
 __device__ __forceinline__ void better(float &key, int &cur, float k2, int c2)
 {
-    const bool take = (k2 > key) || (k2 == key && c2 < cur);
+    const bool take = (k2 > key) | ((k2 == key) & (c2 < cur));
     key = take ? k2 : key;
     cur = take ? c2 : cur;
 }
@@ -366,7 +366,7 @@ struct Mixed {
                     if (act[r]) {
                         const float v = a[r][jj];
                         const float k2 = (v != v) ? __builtin_huge_valf() : fabsf(v);
-                        const bool take = (k2 > key) || (k2 == key && cur[r] < kc);
+                        const bool take = (k2 > key) | ((k2 == key) & (cur[r] < kc));
                         key = take ? k2 : key;
                         kc = take ? cur[r] : kc;
                         own = take ? r : own;
@@ -391,20 +391,30 @@ struct Mixed {
                     sl.cur = wc;
                 }
                 __syncthreads();
-                float bk = sm.slot[jj & 1][0].key;
-                int bc = sm.slot[jj & 1][0].cur, bw = 0;
+                float ck[kWaves];
+                int cc4[kWaves];
+#pragma unroll
+                for (int q = 0; q < kWaves; ++q) {
+                    ck[q] = sm.slot[jj & 1][q].key;
+                    cc4[q] = sm.slot[jj & 1][q].cur;
+                }
+                asm volatile("" ::: "memory");
+                float bk = ck[0];
+                int bc = cc4[0], bw = 0;
 #pragma unroll
                 for (int q = 1; q < kWaves; ++q) {
-                    const float k2 = sm.slot[jj & 1][q].key;
-                    const int c2 = sm.slot[jj & 1][q].cur;
-                    const bool take = (k2 > bk) || (k2 == bk && c2 < bc);
-                    bk = take ? k2 : bk;
-                    bc = take ? c2 : bc;
+                    const bool take = (ck[q] > bk) | ((ck[q] == bk) & (cc4[q] < bc));
+                    bk = take ? ck[q] : bk;
+                    bc = take ? cc4[q] : bc;
                     bw = take ? q : bw;
                 }
                 const FSlot &ps = sm.slot[jj & 1][bw];
                 const int prow = ps.row;
-                const float piv = ps.val[jj];
+                float u[CH];
+#pragma unroll
+                for (int cc = jj; cc < CH; ++cc) u[cc] = ps.val[cc];
+                asm volatile("" ::: "memory");
+                const float piv = u[jj];
                 if (!(piv != 0.0f) || piv != piv) bad = 1;
                 if (tid == 0) {
                     sm.p[gpos] = prow;
@@ -424,7 +434,7 @@ struct Mixed {
                         const float l = a[r][jj] * rc;
                         a[r][jj] = l;
 #pragma unroll
-                        for (int cc = jj + 1; cc < CH; ++cc) a[r][cc] = fmaf(l, -ps.val[cc], a[r][cc]);
+                        for (int cc = jj + 1; cc < CH; ++cc) a[r][cc] = fmaf(l, -u[cc], a[r][cc]);
                     }
                 }
             }
@@ -463,7 +473,11 @@ struct Mixed {
                     if (i >= jj && i < kb) sm.ccol[jj & 1][i] = a[r][jj];
                 }
                 __syncthreads();
-                const float d = sm.ccol[jj & 1][jj];
+                float cv[CH];
+#pragma unroll
+                for (int cc = jj; cc < CH; ++cc) cv[cc] = sm.ccol[jj & 1][cc];
+                asm volatile("" ::: "memory");
+                const float d = cv[jj];
                 if (!(d > 0.0f)) bad = 1;
                 const float ljj = sqrtf(d);
                 const float rl = 1.0f / ljj;
@@ -476,7 +490,7 @@ struct Mixed {
                         a[r][jj] = l;
 #pragma unroll
                         for (int cc = jj + 1; cc < CH; ++cc)
-                            if (cc < kb) a[r][cc] = fmaf(-l, sm.ccol[jj & 1][cc] * rl, a[r][cc]);
+                            if (cc < kb) a[r][cc] = fmaf(-l, cv[cc] * rl, a[r][cc]);
                     }
                 }
             }
@@ -609,15 +623,20 @@ struct Mixed {
                 sm.u.s.T[i][k] = v;
             }
             __syncthreads();
+            float trow[CH];
+#pragma unroll
+            for (int k = 0; k < CH; ++k) trow[k] = sm.u.s.T[lane & (CH - 1)][k];
+            const float tdiag = sm.u.s.T[lane & (CH - 1)][lane & (CH - 1)];
+            asm volatile("" ::: "memory");
             for (int g = w; g < nr; g += kWaves) {
                 const float v0 = sm.u.s.v[g][sm.p[min(kq + lane, NMAX - 1)]];
                 float vi = (lane < kb) ? v0 : 0.0f;
 #pragma unroll
                 for (int k = 0; k < CH; ++k) {
                     if (k < kb) {
-                        if (chol && lane == k) vi = vi / sm.u.s.T[k][k];
+                        if (chol && lane == k) vi = vi / tdiag;
                         const float zk = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(vi), k));
-                        if (lane > k && lane < kb) vi = fmaf(-sm.u.s.T[lane & (CH - 1)][k], zk, vi);
+                        if (lane > k && lane < kb) vi = fmaf(-trow[k], zk, vi);
                     }
                 }
                 if (lane < kb) {
@@ -655,15 +674,20 @@ struct Mixed {
                 sm.u.s.T[i][k] = v;
             }
             __syncthreads();
+            float trow[CH];
+#pragma unroll
+            for (int k = 0; k < CH; ++k) trow[k] = sm.u.s.T[lane & (CH - 1)][k];
+            const float tdiag = sm.u.s.T[lane & (CH - 1)][lane & (CH - 1)];
+            asm volatile("" ::: "memory");
             for (int g = w; g < nr; g += kWaves) {
                 const float v0 = sm.u.s.v[g][sm.p[min(kq + lane, NMAX - 1)]];
                 float vi = (lane < kb) ? v0 : 0.0f;
 #pragma unroll
                 for (int k = CH - 1; k >= 0; --k) {
                     if (k < kb) {
-                        if (lane == k) vi = vi / sm.u.s.T[k][k];
+                        if (lane == k) vi = vi / tdiag;
                         const float yk = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(vi), k));
-                        if (lane < k) vi = fmaf(-sm.u.s.T[lane & (CH - 1)][k], yk, vi);
+                        if (lane < k) vi = fmaf(-trow[k], yk, vi);
                     }
                 }
                 if (lane < kb) {
@@ -735,12 +759,20 @@ struct Mixed {
                 for (int g = 0; g < RG; ++g) acc[g] = (g < nr) ? B[o + (size_t)g * ldb] : 0.0;
                 double rs = 0.0;
                 const double *arow = A + o;
-                for (int c = 0; c < n; ++c) {
-                    const double a = arow[(size_t)c * lda];
-                    rs += fabs(a);
+                // eight independent loads in flight per step
+#pragma unroll 1
+                for (int c0 = 0; c0 < n; c0 += 8) {
+                    double a8[8];
 #pragma unroll
-                    for (int g = 0; g < RG; ++g)
-                        if (g < nr) acc[g] = fma(-a, sm.u.s.x[g][c], acc[g]);
+                    for (int t = 0; t < 8; ++t) a8[t] = (c0 + t < n) ? arow[(size_t)(c0 + t) * lda] : 0.0;
+#pragma unroll
+                    for (int t = 0; t < 8; ++t) {
+                        const int c = min(c0 + t, NMAX - 1);
+                        rs += fabs(a8[t]);
+#pragma unroll
+                        for (int g = 0; g < RG; ++g)
+                            if (g < nr) acc[g] = fma(-a8[t], sm.u.s.x[g][c], acc[g]);
+                    }
                 }
                 rs_max = dmax_nan(rs_max, rs);
 #pragma unroll

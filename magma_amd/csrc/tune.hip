@@ -75,7 +75,16 @@ struct DevBuf {
     void *p = nullptr;
     ~DevBuf() { if (p) (void)hipFree(p); }
     template <class T> T *get() { return static_cast<T *>(p); }
-    bool alloc(size_t b) { return hipMalloc(&p, b) == hipSuccess; }
+    bool alloc(size_t b)
+    {
+        const hipError_t e = hipMalloc(&p, b);
+        if (e != hipSuccess) {
+            std::fprintf(stderr, "magma_amd_autotune: hipMalloc(%zu) failed: %s\n", b, hipGetErrorString(e));
+            p = nullptr;
+            return false;
+        }
+        return true;
+    }
 };
 
 // median-free: best of `reps` timings of fn() (each preceded by prep())

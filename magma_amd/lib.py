@@ -169,8 +169,16 @@ _lib = None
 
 
 def load():
-    """Return the process-wide MagmaLibrary (loads it on first use)."""
+    """Return the process-wide MagmaLibrary (loads it on first use).
+
+    torch, when installed, is imported first: torch ships its own HIP runtime
+    under the same soname as the system one, and a process must use a single
+    runtime so that torch's streams and allocations are valid in the library."""
     global _lib
     if _lib is None:
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         _lib = MagmaLibrary()
     return _lib

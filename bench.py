@@ -209,7 +209,8 @@ def workload_cond256(args, torch, ma, L, q):
     from magma_amd import gen
     count, n, nrhs = args.batch or 8192, 256, 1
     log(f"generating {count} conditioned matrices n={n}")
-    A0, kappa = gen.cond_batch_torch(3, count, n, "cuda")
+    A0, kappa = gen.cond_batch_torch(
+        3, count, n, "cuda", progress=lambda d, t: d % 2048 == 0 and log(f"  generated {d}/{t}"))
     log("inputs ready")
     B0 = gen.uniform_pm1_torch(103, 0, count, n * nrhs, "cuda").view(count, nrhs, n).contiguous()
     ipiv = ma.IntBatch(count, n)
@@ -542,6 +543,14 @@ def main():
     ap.add_argument("--batch", type=int, default=0, help="override the batch count (cond256 only)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--serve-seconds", type=float, default=30.0, help="trace length of serve_poisson")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run one 1 GiB device-to-device copy (the reference kernel for scaling "
+                         "FETCH_SIZE / WRITE_SIZE counters to bytes)")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="skip magma_amd_autotune and use the untuned kernel variants")
+    ap.add_argument("--tune-table", default="",
+                    help="load the variant table from this file if it exists (no tuning run), "
+                         "else tune and save it there")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", dest="check", action="store_false")
     args = ap.parse_args()
@@ -556,6 +565,31 @@ def main():
     q = ma.Queue(0)
     L.magma_queue_set_async(q.handle, 1)      # back-to-back submission; synchronize around the timed region
 
+    if args.calibrate_copy:
+        # an elementwise kernel of known traffic: reads 1 GiB, writes 1 GiB
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
+        log("calibration kernel: 1 GiB read + 1 GiB written")
+    tuned = None
+    if args.autotune:
+        t0 = time.perf_counter()
+        if args.tune_table and os.path.exists(args.tune_table):
+            rc = L.magma_amd_tune_load(args.tune_table.encode())
+            if rc != 0:
+                raise RuntimeError(f"magma_amd_tune_load({args.tune_table}) failed: {rc}")
+        else:
+            rc = L.magma_amd_autotune(q.handle, 0)
+            if rc != 0:
+                raise RuntimeError(f"magma_amd_autotune failed: {rc}")
+            if args.tune_table and L.magma_amd_tune_save(args.tune_table.encode()) != 0:
+                raise RuntimeError(f"magma_amd_tune_save({args.tune_table}) failed")
+        tuned = {name: [L.magma_amd_tune_get(fam, c) for c in range(10)]
+                 for fam, name in enumerate(("getrf", "mixed", "gemm"))}
+        log(f"autotune {time.perf_counter() - t0:.1f} s: {tuned}")
     log(f"workload {args.workload}")
     r = WORKLOADS[args.workload](args, torch, ma, L, q)
     roof = r["roofline"]
@@ -579,7 +613,7 @@ def main():
         "vs_baseline": None,
         "dtype": r["dtype"],
         "data": "synthetic (deterministic splitmix64 generator, inputs resident in HBM)",
-        "config": dict(r["config"], parallelism="single GPU, batch-parallel", **lat),
+        "config": dict(r["config"], parallelism="single GPU, batch-parallel", tuned_variants=tuned, **lat),
         "roofline": roof,
         "cpu_baseline": cpu,
         "check": r["check"],

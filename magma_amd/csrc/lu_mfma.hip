@@ -228,12 +228,23 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(M
                             double l = a[i][jj];
                             if (piv != 0.0) l = use_rcp ? rc * l : l / piv;
                             a[i][jj] = l;
-                            bad |= !__builtin_isfinite(l);
+                            const bool fin = __builtin_isfinite(l);
+                            bad |= !fin;
+                            if (__builtin_amdgcn_ballot_w64(!fin) == 0) {
+                                // finite multipliers: a zero pivot-row entry adds
+                                // a zero, so the reference dger's skip of it
+                                // changes nothing (up to the sign of a zero)
 #pragma unroll
-                            for (int c = jj + 1; c < PW; ++c) {
-                                // reference dger: a zero pivot-row entry is skipped
-                                const double v = fma(l, -u[c], a[i][c]);
-                                a[i][c] = (u[c] != 0.0) ? v : a[i][c];
+                                for (int c = jj + 1; c < PW; ++c) a[i][c] = fma(l, -u[c], a[i][c]);
+                            } else {
+                                // reference dger: a zero pivot-row entry is
+                                // skipped.  fma(0, -0, x) is x exactly, so a zero
+                                // multiplier there leaves the entry as it was.
+                                double ls = l;
+                                asm volatile("" : "+v"(ls));      // keeps the two loops apart
+#pragma unroll
+                                for (int c = jj + 1; c < PW; ++c)
+                                    a[i][c] = fma((u[c] != 0.0) ? ls : 0.0, -u[c], a[i][c]);
                             }
                         }
                     }

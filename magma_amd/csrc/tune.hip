@@ -175,11 +175,11 @@ magma_int_t tune_getrf(hipStream_t st, FILE *log)
 
 magma_int_t tune_mixed(hipStream_t st, FILE *log)
 {
-    const int classes[] = {5, 6};
-    const int dims[] = {192, 384};
-    const int counts[] = {1024, 256};
-    const int nv[] = {3, 2};
-    for (int t = 0; t < 2; ++t) {
+    const int classes[] = {4, 5, 6};
+    const int dims[] = {96, 192, 384};
+    const int counts[] = {4096, 1024, 256};
+    const int nv[] = {3, 3, 2};
+    for (int t = 0; t < 3; ++t) {
         const int n = dims[t], count = counts[t];
         const size_t elems = (size_t)n * n * count;
         DevBuf a, b, x, pa, pb, px, w, it, inf, fb;

@@ -116,8 +116,11 @@ def cond_batch(seed, batch, n, first=0):
     G = normal(seed, mats, 2 * n * n)
     G1 = G[:, : n * n].reshape(batch, n, n)       # storage order: column-major
     G2 = G[:, n * n:].reshape(batch, n, n)
-    U, _ = np.linalg.qr(np.swapaxes(G1, -1, -2))
-    V, _ = np.linalg.qr(np.swapaxes(G2, -1, -2))
+    U, RU = np.linalg.qr(np.swapaxes(G1, -1, -2))
+    V, RV = np.linalg.qr(np.swapaxes(G2, -1, -2))
+    # the unique QR: R's diagonal positive
+    U = U * np.sign(np.diagonal(RU, axis1=-2, axis2=-1))[..., None, :]
+    V = V * np.sign(np.diagonal(RV, axis1=-2, axis2=-1))[..., None, :]
     kappa, s = cond_batch_params(seed, batch, n, first)
     A = (U * s[:, None, :]) @ np.swapaxes(V, -1, -2)
     return to_colmajor(A), kappa
@@ -166,28 +169,47 @@ def normal_torch(seed, first, batch, count, device, elem0=0):
     return torch.sqrt(-2.0 * torch.log1p(-u1)) * torch.cos(2.0 * torch.pi * u2)
 
 
+def _cholqr2(G):
+    """Q of G = QR with R's diagonal positive, by Cholesky QR applied twice
+    (Q1 = G chol(G'G)^-1, then the same on Q1): orthogonal to rounding for
+    cond(G) well below 1e7, which Gaussian matrices of these sizes are."""
+    import torch
+    Q = G
+    for _ in range(2):
+        R = torch.linalg.cholesky(Q.transpose(-1, -2) @ Q, upper=True)
+        Q = torch.linalg.solve_triangular(R, Q, upper=True, left=False)
+    return Q
+
+
 def _q_factor(G, state):
-    """Q of the QR of each matrix of G (rows, cols), on G's device when the
-    device solver accepts the batch, else on the host.  The batched device QR
-    can refuse large batches for want of solver workspace (a library status,
-    not a device error); the host QR then serves the remaining chunks."""
+    """Q of the QR of each matrix of G (rows, cols) with R's diagonal positive,
+    on G's device when the device solvers accept the batch, else on the host.
+    A batched device solver can refuse a batch for want of workspace (a
+    library status, not a device error); the host QR then serves the
+    remaining chunks."""
     import torch
     if state.get("device_qr", True):
         try:
-            return torch.linalg.qr(G.contiguous())[0]
+            return _cholqr2(G.contiguous())
         except RuntimeError:
             state["device_qr"] = False
-    return torch.linalg.qr(G.cpu().contiguous())[0].to(G.device)
+    Q, R = torch.linalg.qr(G.cpu().contiguous())
+    Q = Q * torch.sign(torch.diagonal(R, dim1=-2, dim2=-1))[..., None, :]
+    return Q.to(G.device)
 
 
-def cond_batch_torch(seed, batch, n, device, first=0, chunk=128):
-    """cfg4 matrices built on the device: (batch, n, n) column-major storage.
-    Same construction as cond_batch (QR of N(0,1) matrices)."""
+def cond_batch_torch(seed, batch, n, device, first=0, chunk=128, progress=None):
+    """Conditioned matrices built on the device: (batch, n, n) column-major
+    storage.  Same construction as cond_batch (QR of N(0,1) matrices, R's
+    diagonal positive); equal to it up to rounding.  progress(done, total) is
+    called after each chunk."""
     import torch
     out = torch.empty((batch, n, n), dtype=torch.float64, device=device)
     kappa_all = []
     state = {}
     for c0 in range(0, batch, chunk):
+        if progress:
+            progress(c0, batch)
         b = min(chunk, batch - c0)
         G1 = normal_torch(seed, first + c0, b, n * n, device).view(b, n, n)       # storage (cols, rows)
         G2 = normal_torch(seed, first + c0, b, n * n, device, elem0=n * n).view(b, n, n)

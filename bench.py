@@ -84,6 +84,22 @@ class Timer:
         return [a.elapsed_time(b) for a, b in self.pairs]
 
 
+def measured_traffic(workload, kernel, algorithmic_bytes):
+    """HBM bytes per launch of `kernel` measured by rocprofv3 --pmc FETCH_SIZE
+    and WRITE_SIZE (separate runs), each scaled by the same counter's reading
+    on a calibration kernel of known traffic; read from the committed profile
+    summary (profiles/hbm_traffic.json).  {} when none is recorded."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as fh:
+            rec = json.load(fh)[workload][kernel]
+    except (OSError, KeyError, ValueError):
+        return {}
+    t = rec["fetch_kib"] * 1024 * rec["fetch_scale"] + rec["write_kib"] * 1024 * rec["write_scale"]
+    return {"traffic": t, "traffic_over_algorithmic": t / algorithmic_bytes,
+            "traffic_source": "profiles/hbm_traffic.json (" + rec["source"] + ")"}
+
+
 def log(msg):
     """Progress line on stderr (stdout carries only the JSON result)."""
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
@@ -286,8 +302,10 @@ def workload_cond256(args, torch, ma, L, q):
             check["mixed_max_backward_err_ratio_to_n_eps"] = float(max(errs))
         byt = count * (8.0 * n * n + 16.0 * n * nrhs + 8)
         roof = {"bound": "hbm", "achieved": byt / (res["mixed"]["kernel_ms"] * 1e-3) / 1e9,
-                "peak": HBM_PEAK / 1e9, "unit": "GB/s", "kernel": "dsgesv (mixed factor + FP64 refinement)",
+                "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "kernel": "mixed_kernel (FP32 factor + FP64 refinement; the dsgesv call's dominant launch)",
                 "kernel_ms": res["mixed"]["kernel_ms"], "bytes_per_launch": byt}
+        roof.update(measured_traffic("cond256_mixed_vs_plain", "mixed_kernel", byt))
     return dict(value=value, unit="solves/s", ms=ms, dtype="f64",
                 config={"workload": "cond256_mixed_vs_plain", "batch": count, "n": n, "nrhs": nrhs,
                         "inputs": "A = U diag(s) V', U,V from QR of N(0,1) (seed 3), cond log-uniform [1e2,1e6]",

@@ -12,8 +12,15 @@ constexpr int kMixedMaxN = 512;
 
 // Per-matrix factor storage, in floats: the n x n factor (ld n, rows in the
 // input's original order), then n ints of scaling exponents, then n ints of
-// pivot order (original row of pivot position i).
-__host__ __device__ inline size_t mixed_factor_floats(int n) { return (size_t)n * n + 2 * (size_t)n; }
+// pivot order (original row of pivot position i), then (LU) the diagonal
+// 32 x 32 block of every 32-column chunk with its rows in pivot order,
+// [chunk][i][k] = factor(p[32 chunk + i], 32 chunk + k), so that the block
+// solves read it as one contiguous 4 KB piece instead of 32 scattered rows.
+__host__ __device__ inline size_t mixed_diag_offset(int n) { return ((size_t)n * n + 2 * (size_t)n + 3) & ~(size_t)3; }
+__host__ __device__ inline size_t mixed_factor_floats(int n)
+{
+    return mixed_diag_offset(n) + 1024 * (size_t)((n + 31) / 32);
+}
 
 struct MixedSet {
     int n = 0, nrhs = 0, lda = 0, ldb = 0, ldx = 0;       // fixed sizes

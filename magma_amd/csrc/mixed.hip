@@ -236,12 +236,10 @@ struct Mixed {
         const int tid = threadIdx.x;
         if (!chol) {
             for (int e = tid; e < CH * CH; e += kT) {
-                const int i = e % CH, i2 = e / CH;
+                const int i = e / CH, i2 = e % CH;
                 float v = 0.0f;
-                if (i2 < i && i < kb) {
-                    const int o = sm.p[kq + i];
-                    v = Fg ? Fg[o + (size_t)(kq + i2) * n] : sm.u.f.blk[kl0 + i2][o];
-                }
+                if (i2 < i && i < kb)
+                    v = Fg ? Fg[mixed_diag_offset(n) + 32 * (size_t)kq + CH * i + i2] : sm.u.f.blk[kl0 + i2][sm.p[kq + i]];
                 sm.u.f.L11[i][i2] = v;
             }
             __syncthreads();
@@ -517,6 +515,15 @@ struct Mixed {
                 const int cc = e / n, o = e % n;
                 F[o + (size_t)(c0 + cc) * n] = sm.u.f.blk[cc][o];
             }
+            // the diagonal blocks of the block column's chunks, rows in pivot order
+            float *Fd = F + mixed_diag_offset(n) + 32 * (size_t)c0;
+            const int nbr = (nb + CH - 1) / CH * CH;
+            for (int e = tid; e < nbr * CH; e += kT) {
+                const int s0 = (e / (CH * CH)) * CH, i = (e / CH) % CH, k = e % CH;
+                float v = 0.0f;
+                if (s0 + i < nb && s0 + k < nb) v = sm.u.f.blk[s0 + k][sm.p[c0 + s0 + i]];
+                Fd[e] = v;
+            }
         } else {
             // lower part in place; the transpose into the upper part (used by the
             // backward sweep as an upper-triangular factor)
@@ -617,9 +624,11 @@ struct Mixed {
         for (int kq = 0; kq < n; kq += CH) {
             const int kb = min(CH, n - kq);
             for (int e = tid; e < CH * CH; e += kT) {
-                const int i = e % CH, k = e / CH;
+                // consecutive threads: down a column (Cholesky) / along a block row (LU)
+                const int i = chol ? e % CH : e / CH, k = chol ? e / CH : e % CH;
                 float v = 0.0f;
-                if (i < kb && k < kb && k <= i) v = F[sm.p[kq + i] + (size_t)(kq + k) * n];
+                if (i < kb && k < kb && k <= i)
+                    v = chol ? F[(kq + i) + (size_t)(kq + k) * n] : F[mixed_diag_offset(n) + 32 * (size_t)kq + CH * i + k];
                 sm.u.s.T[i][k] = v;
             }
             __syncthreads();
@@ -662,15 +671,16 @@ struct Mixed {
             }
             __syncthreads();
         }
-        // backward
+        // backward, over the same 32-column chunks as the factorization
 #pragma unroll 1
-        for (int kend = n; kend > 0; kend -= CH) {
-            const int kq = max(0, kend - CH);
-            const int kb = kend - kq;
+        for (int kq = ((n - 1) / CH) * CH; kq >= 0; kq -= CH) {
+            const int kb = min(CH, n - kq);
             for (int e = tid; e < CH * CH; e += kT) {
-                const int i = e % CH, k = e / CH;
+                // consecutive threads: down a column (Cholesky) / along a block row (LU)
+                const int i = chol ? e % CH : e / CH, k = chol ? e / CH : e % CH;
                 float v = 0.0f;
-                if (i < kb && k < kb && k >= i) v = F[sm.p[kq + i] + (size_t)(kq + k) * n];
+                if (i < kb && k < kb && k >= i)
+                    v = chol ? F[(kq + i) + (size_t)(kq + k) * n] : F[mixed_diag_offset(n) + 32 * (size_t)kq + CH * i + k];
                 sm.u.s.T[i][k] = v;
             }
             __syncthreads();
@@ -939,7 +949,9 @@ static int resident_per_cu(K kernel)
 }
 
 // Kernel variants (block-column width NB, right-hand-side group RG):
-//   n <= 256: 0 (NB 128, RG 16), 1 (NB 64, RG 8), 2 (NB 32, RG 8)
+//   n <= 256: 0 (NB 128, RG 16), 1 (NB 64, RG 8), 2 (NB 32, RG 8), 3 (NB 32,
+//             RG 8, one workgroup per CU: the workgroups' matrices and factors
+//             then stay within the memory-side cache between their passes)
 //   n <= 512: 0 (NB 64, RG 8), 1 (NB 32, RG 8)
 static int mixed_variant(int max_n)
 {
@@ -952,7 +964,8 @@ int mixed_grid(int count, int max_n)
     int per_cu;
     if (max_n <= 256) {
         per_cu = v == 0 ? resident_per_cu(mixed_kernel<256, 128, 16>)
-               : v == 1 ? resident_per_cu(mixed_kernel<256, 64, 8>) : resident_per_cu(mixed_kernel<256, 32, 8>);
+               : v == 1 ? resident_per_cu(mixed_kernel<256, 64, 8>)
+               : v == 2 ? resident_per_cu(mixed_kernel<256, 32, 8>) : 1;
     } else {
         per_cu = v == 0 ? resident_per_cu(mixed_kernel<512, 64, 8>) : resident_per_cu(mixed_kernel<512, 32, 8>);
     }

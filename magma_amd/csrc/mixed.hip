@@ -42,6 +42,7 @@
 #include <cstdio>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "magma_amd.h"
 #include "mixed.h"
@@ -82,7 +83,11 @@ struct MxSmem {
             float blk[NB][NMAX + 1];          // block column, [column][original row]
             _Float16 Uh[NB][kUPad];           // -U (LU) or -L^T (Cholesky) of the chunk, split
             _Float16 Ul[NB][kUPad];
-            float L11[CH][CH + 1];
+            // L11 transposed: LT[k][r] = L(r, k) for r > k, zero elsewhere
+            // (r up to 2 CH - 2, so column k from row k + 1 on is one run of
+            // CH - 1 entries at constant offsets); the odd pitch keeps the
+            // transposing stores free of bank conflicts
+            float LT[CH][2 * CH - 1];
         } f;
         struct {
             double x[RG][NMAX];
@@ -169,6 +174,31 @@ __device__ __forceinline__ void wave_argmax(float &key, int &cur)
     cur = c;
 }
 
+template <int I, int N, class Fn>
+__device__ __forceinline__ void static_for(Fn &&fn)
+{
+    if constexpr (I < N) {
+        fn(std::integral_constant<int, I>{});
+        static_for<I + 1, N>(fn);
+    }
+}
+
+// Lanes 0-7 of each row of 16 get lane G's value, lanes 8-15 lane 8 + G's
+// (row_newbcast: every lane of a row reads the given lane of its row).
+template <int G>
+__device__ __forceinline__ float bcast8(float v, int lane)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x150 + G, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x158 + G, 0xF, 0xF, false);
+    return __int_as_float((lane & 8) ? hi : lo);
+}
+
+// raw buffer resource over [p, p + bytes) (gfx9 descriptor word 3)
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void *p, unsigned bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, (int)bytes, 0x00020000);
+}
+
 __device__ __forceinline__ void split(float v, _Float16 &h, _Float16 &l)
 {
     h = (_Float16)v;
@@ -235,39 +265,50 @@ struct Mixed {
     {
         const int tid = threadIdx.x;
         if (!chol) {
-            for (int e = tid; e < CH * CH; e += kT) {
-                const int i = e / CH, i2 = e % CH;
+            for (int e = tid; e < (2 * CH - 1) * CH; e += kT) {
+                const int i = e / CH, i2 = e % CH;      // row i (< 2 CH - 1), column i2
                 float v = 0.0f;
                 if (i2 < i && i < kb)
                     v = Fg ? Fg[mixed_diag_offset(n) + 32 * (size_t)kq + CH * i + i2] : sm.u.f.blk[kl0 + i2][sm.p[kq + i]];
-                sm.u.f.L11[i][i2] = v;
+                sm.u.f.LT[i2][i] = v;
             }
             __syncthreads();
-            // ce - cs <= NB <= kT: one column per thread.  sm.p is a valid row
-            // index at every position, so the loads need no guard.
-            const int cc = cs + tid;
-            if (cc < ce) {
-                // x[j] holds row k + j of the column at step k; the loop over k is
-                // kept rolled (registers rotate instead of being indexed by k)
-                float x[CH];
-#pragma unroll
-                for (int i = 0; i < CH; ++i) {
-                    const float v = sm.u.f.blk[cc][sm.p[min(kq + i, NMAX - 1)]];
-                    x[i] = (i < kb) ? v : 0.0f;
-                }
+            // U of the chunk's pivot rows: eight threads per column, four rows
+            // each, all 32 steps unrolled.  At step k the row-k value of a
+            // column is final; it is broadcast to the column's other seven
+            // lanes by DPP (row_newbcast within each row of 16 lanes, which
+            // holds two columns), and every row r > k takes fma(L(r,k), -x_k).
+            // LT is zero at r <= k and r >= kb, so those rows are unchanged.
+            // sm.p is a valid row index at every position, so loads need no guard.
+            const int lane = tid & 63, g = tid & 7;
 #pragma unroll 1
-                for (int k = 0; k < kb; ++k) {
-                    const float xk = x[0];
-                    sm.u.f.blk[cc][sm.p[kq + k]] = xk;
-                    split(-xk, sm.u.f.Uh[cc][k], sm.u.f.Ul[cc][k]);
-                    // rows past the chunk (k + i >= CH) are never output
+            for (int cb = cs; cb < ce; cb += kT / 8) {
+                const int cc = cb + (tid >> 3);
+                const int ccl = cc < ce ? cc : cs;
+                float x[4], lc[4], ln[4];
 #pragma unroll
-                    for (int i = 1; i < CH; ++i) x[i - 1] = fmaf(sm.u.f.L11[min(k + i, CH - 1)][k], -xk, x[i]);
-                    x[CH - 1] = 0.0f;
+                for (int j = 0; j < 4; ++j) {
+                    const float v = sm.u.f.blk[ccl][sm.p[kq + 4 * g + j]];
+                    x[j] = (4 * g + j < kb) ? v : 0.0f;
+                    lc[j] = sm.u.f.LT[0][4 * g + j];
                 }
-                for (int k = kb; k < CH; ++k) {
-                    sm.u.f.Uh[cc][k] = (_Float16)0.0f;
-                    sm.u.f.Ul[cc][k] = (_Float16)0.0f;
+                static_for<0, CH>([&](auto kc) {
+                    constexpr int k = decltype(kc)::value;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) ln[j] = sm.u.f.LT[k + 1 < CH ? k + 1 : CH - 1][4 * g + j];
+                    const float xk = bcast8<(k >> 2)>(x[k & 3], lane);
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) x[j] = fmaf(lc[j], -xk, x[j]);
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) lc[j] = ln[j];
+                });
+                if (cc < ce) {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const int r = 4 * g + j;
+                        if (r < kb) sm.u.f.blk[cc][sm.p[kq + r]] = x[j];
+                        split((r < kb) ? -x[j] : 0.0f, sm.u.f.Uh[cc][r], sm.u.f.Ul[cc][r]);
+                    }
                 }
             }
         } else {
@@ -291,6 +332,10 @@ struct Mixed {
         const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
         const int li = lane & 15, lk = lane >> 4;
         const int ostart = (rlo / 16) * 16;
+        if (Fg) {
+            gemm_stored(sm, n, kq, kb, cs, ce, Fg, rlo);
+            return;
+        }
 #pragma unroll 1
         for (int o0 = ostart + 16 * w; o0 < n; o0 += 16 * kWaves) {
             const int o = o0 + li;
@@ -303,6 +348,77 @@ struct Mixed {
                     const int k = 16 * ks + 4 * lk + t;
                     float v = 0.0f;
                     if (act && k < kb) v = Fg ? Fg[o + (size_t)(kq + k) * n] : sm.u.f.blk[kl0 + k][o];
+                    _Float16 h, l;
+                    split(v, h, l);
+                    lh[ks][t] = h;
+                    ll[ks][t] = l;
+                }
+            }
+#pragma unroll 1
+            for (int cb = cs; cb < ce; cb += 16) {
+                float4v acc;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int c = cb + 4 * lk + i;
+                    acc[i] = (c < ce) ? sm.u.f.blk[c][o] : 0.0f;
+                }
+#pragma unroll
+                for (int ks = 0; ks < (CH / 16); ++ks) {
+                    const half4 uh = *reinterpret_cast<const half4 *>(&sm.u.f.Uh[cb + li][16 * ks + 4 * lk]);
+                    const half4 ul = *reinterpret_cast<const half4 *>(&sm.u.f.Ul[cb + li][16 * ks + 4 * lk]);
+                    acc = __builtin_amdgcn_mfma_f32_16x16x16f16(ul, lh[ks], acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_16x16x16f16(uh, ll[ks], acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_16x16x16f16(uh, lh[ks], acc, 0, 0, 0);
+                }
+                if (act) {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const int c = cb + 4 * lk + i;
+                        if (c < ce) sm.u.f.blk[c][o] = acc[i];
+                    }
+                }
+            }
+        }
+    }
+
+    // The same product with L from the factor storage: every L value of the
+    // wavefront's row blocks is loaded first (one round of memory latency per
+    // chunk instead of one per row block), by buffer loads whose column part
+    // is a scalar offset.  Loads past column n - 1 (a partial last chunk) stay
+    // inside the factor storage and are discarded.
+    static constexpr int RB = NMAX / (16 * kWaves);     // row blocks per wavefront
+    static __device__ __forceinline__ void gemm_stored(Smem &sm, int n, int kq, int kb, int cs, int ce, const float *Fg,
+                                                       int rlo)
+    {
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        const int li = lane & 15, lk = lane >> 4;
+        const int ob = (rlo / 16) * 16 + 16 * w;
+        const __amdgpu_buffer_rsrc_t Fr = rsrc(Fg, 4u * (unsigned)mixed_factor_floats(n));
+        float lv[RB][8];
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) {
+            const int o = min(ob + 16 * kWaves * rb + li, n - 1);
+            const int vo = 4 * (o + 4 * lk * n);
+#pragma unroll
+            for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    lv[rb][4 * ks + t] =
+                        __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(Fr, vo, 4 * n * (kq + 16 * ks + t), 0));
+        }
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) {
+            const int o0 = ob + 16 * kWaves * rb;
+            if (o0 >= n) break;
+            const int o = o0 + li;
+            const bool act = o < n && o >= rlo && sm.pivpos[o] >= kq + kb;
+            half4 lh[2], ll[2];
+#pragma unroll
+            for (int ks = 0; ks < 2; ++ks) {
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int k = 16 * ks + 4 * lk + t;
+                    const float v = (act && k < kb) ? lv[rb][4 * ks + t] : 0.0f;
                     _Float16 h, l;
                     split(v, h, l);
                     lh[ks][t] = h;
@@ -849,8 +965,10 @@ struct Mixed {
     }
 };
 
-template <int NMAX, int NB, int RG>
-__global__ void __launch_bounds__(kT) mixed_kernel(MixedSet g)
+// OCC: wavefronts per SIMD the register allocation is held to (workgroups
+// per CU when the LDS allows it)
+template <int NMAX, int NB, int RG, int OCC>
+__global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(OCC))) mixed_kernel(MixedSet g)
 {
     using M = Mixed<NMAX, NB, RG>;
     __shared__ typename M::Smem sm;
@@ -949,9 +1067,11 @@ static int resident_per_cu(K kernel)
 }
 
 // Kernel variants (block-column width NB, right-hand-side group RG):
-//   n <= 256: 0 (NB 128, RG 16), 1 (NB 64, RG 8), 2 (NB 32, RG 8), 3 (NB 32,
-//             RG 8, one workgroup per CU: the workgroups' matrices and factors
-//             then stay within the memory-side cache between their passes)
+//   n <= 256: 0 (NB 128, RG 16), 1 (NB 64, RG 8), 2 (NB 32, RG 8, two
+//             workgroups per CU), 3 (as 2 but one workgroup per CU: the
+//             workgroups' matrices and factors then stay within the
+//             memory-side cache between their passes), 4 (as 2 with registers
+//             held to three workgroups per CU)
 //   n <= 512: 0 (NB 64, RG 8), 1 (NB 32, RG 8)
 static int mixed_variant(int max_n)
 {
@@ -963,11 +1083,12 @@ int mixed_grid(int count, int max_n)
     const int v = mixed_variant(max_n);
     int per_cu;
     if (max_n <= 256) {
-        per_cu = v == 0 ? resident_per_cu(mixed_kernel<256, 128, 16>)
-               : v == 1 ? resident_per_cu(mixed_kernel<256, 64, 8>)
-               : v == 2 ? resident_per_cu(mixed_kernel<256, 32, 8>) : 1;
+        per_cu = v == 0 ? resident_per_cu(mixed_kernel<256, 128, 16, 1>)
+               : v == 1 ? resident_per_cu(mixed_kernel<256, 64, 8, 1>)
+               : v == 2 ? resident_per_cu(mixed_kernel<256, 32, 8, 2>)
+               : v == 3 ? 1 : resident_per_cu(mixed_kernel<256, 32, 8, 3>);
     } else {
-        per_cu = v == 0 ? resident_per_cu(mixed_kernel<512, 64, 8>) : resident_per_cu(mixed_kernel<512, 32, 8>);
+        per_cu = v == 0 ? resident_per_cu(mixed_kernel<512, 64, 8, 1>) : resident_per_cu(mixed_kernel<512, 32, 8, 1>);
     }
     return persistent_grid(count, per_cu);
 }
@@ -979,12 +1100,13 @@ magma_int_t launch_mixed(const MixedSet &g, int max_n, hipStream_t st)
     const int grid = mixed_grid(g.count, max_n);
     const int v = mixed_variant(max_n);
     if (max_n <= 256) {
-        if (v == 0) mixed_kernel<256, 128, 16><<<grid, kT, 0, st>>>(g);
-        else if (v == 1) mixed_kernel<256, 64, 8><<<grid, kT, 0, st>>>(g);
-        else mixed_kernel<256, 32, 8><<<grid, kT, 0, st>>>(g);
+        if (v == 0) mixed_kernel<256, 128, 16, 1><<<grid, kT, 0, st>>>(g);
+        else if (v == 1) mixed_kernel<256, 64, 8, 1><<<grid, kT, 0, st>>>(g);
+        else if (v == 4) mixed_kernel<256, 32, 8, 3><<<grid, kT, 0, st>>>(g);
+        else mixed_kernel<256, 32, 8, 2><<<grid, kT, 0, st>>>(g);
     } else {
-        if (v == 0) mixed_kernel<512, 64, 8><<<grid, kT, 0, st>>>(g);
-        else mixed_kernel<512, 32, 8><<<grid, kT, 0, st>>>(g);
+        if (v == 0) mixed_kernel<512, 64, 8, 1><<<grid, kT, 0, st>>>(g);
+        else mixed_kernel<512, 32, 8, 1><<<grid, kT, 0, st>>>(g);
     }
     MAGMA_LAUNCH_CHECK();
     return MAGMA_SUCCESS;

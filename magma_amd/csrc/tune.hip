@@ -31,7 +31,7 @@ void init_table()
     g_init = true;
 }
 
-const int kVariants[kTuneFamilies] = {3, 4, 2};
+const int kVariants[kTuneFamilies] = {3, 5, 2};
 
 __global__ void fill_kernel(double *p, size_t count, unsigned seed)
 {
@@ -178,7 +178,7 @@ magma_int_t tune_mixed(hipStream_t st, FILE *log)
     const int classes[] = {4, 5, 6};
     const int dims[] = {96, 192, 384};
     const int counts[] = {4096, 1024, 256};
-    const int nv[] = {4, 4, 2};
+    const int nv[] = {5, 5, 2};
     for (int t = 0; t < 3; ++t) {
         const int n = dims[t], count = counts[t];
         const size_t elems = (size_t)n * n * count;

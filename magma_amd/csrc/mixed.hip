@@ -218,19 +218,33 @@ struct Mixed {
     static __device__ __forceinline__ void load_block(Smem &sm, const double *A, int lda, int n, int c0, int nb, bool chol)
     {
         const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        // LC columns per wavefront in flight at once (rows lane + 64 r)
+        constexpr int LC = RPL <= 4 ? 4 : 2;
 #pragma unroll 1
-        for (int cc = w; cc < nb; cc += kWaves) {
+        for (int cg = w; cg < nb; cg += LC * kWaves) {
+            double vv[LC][RPL];
+#pragma unroll
+            for (int u = 0; u < LC; ++u) {
+                const double *col = A + (size_t)(c0 + min(cg + kWaves * u, nb - 1)) * lda;
+#pragma unroll
+                for (int r = 0; r < RPL; ++r) {
+                    const int o = lane + 64 * r;
+                    // Cholesky reads the lower part only (upper entries are never used)
+                    const bool in = o < n && (!chol || o >= c0);
+                    vv[u][r] = in ? col[min(o, n - 1)] : 0.0;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < LC; ++u) {
+            const int cc = cg + kWaves * u;
+            if (cc >= nb) break;
             const int c = c0 + cc;
-            const double *col = A + (size_t)c * lda;
             double v[RPL];
             double mx = 0.0;
             bool bad = false;
 #pragma unroll
             for (int r = 0; r < RPL; ++r) {
-                const int o = lane + 64 * r;
-                // Cholesky reads the lower part only (upper entries are never used)
-                const bool in = o < n && (!chol || o >= c0);
-                v[r] = in ? col[o] : 0.0;
+                v[r] = vv[u][r];
                 mx = dmax_nan(mx, fabs(v[r]));
                 bad |= !__builtin_isfinite(v[r]);
             }
@@ -251,6 +265,7 @@ struct Mixed {
             if (lane == 0) {
                 if (!chol) sm.cexp[c] = e;
                 if (anybad) sm.flag = 1;
+            }
             }
         }
     }

@@ -255,6 +255,15 @@ def workload_cond256(args, torch, ma, L, q):
     A_s = A0[:64].cpu().numpy(); B_s = B0[:64].cpu().numpy(); X_s = copies[-1][1][:64].cpu().numpy()
     del copies
     res["plain"] = dict(solves_per_s=count / (plain_ms * 1e-3), ms_per_step=plain_ms, factor_kernel_ms=fact_ms)
+    # the plain factorization against both of its bounds
+    fbytes = count * getrf_bytes(n)
+    res["plain"]["roofline"] = dict(
+        {"kernel": "getrf_mfma_kernel<256>", "fp64_tflops": count * getrf_flops(n) / (fact_ms * 1e-3) / 1e12,
+         "fp64_frac": count * getrf_flops(n) / (fact_ms * 1e-3) / FP64_PEAK,
+         "algorithmic_GBps": fbytes / (fact_ms * 1e-3) / 1e9},
+        **measured_traffic("cond256_mixed_vs_plain", "getrf_mfma_kernel", fbytes))
+    if "traffic" in res["plain"]["roofline"]:
+        res["plain"]["roofline"]["traffic_GBps"] = res["plain"]["roofline"]["traffic"] / (fact_ms * 1e-3) / 1e9
     flops = count * getrf_flops(n)
     roof = {"bound": "mfma", "achieved": flops / (fact_ms * 1e-3) / 1e12, "peak": FP64_PEAK / 1e12,
             "unit": "TFLOP/s", "kernel": "getrf_mfma_kernel<256> (FP64 MFMA trailing update; FP64 peak)",

@@ -763,21 +763,25 @@ struct Mixed {
                 sm.u.s.T[i][k] = v;
             }
             __syncthreads();
+            // unit-diagonal form: lane i holds row i of the block divided by its
+            // diagonal (Cholesky; LU's L is unit) with the diagonal itself
+            // zeroed, and v_i divided the same way, so each step is one
+            // broadcast and one fma with no lane conditions.  Rows and
+            // columns past kb are zero in T, and v is zero there.
+            const int li = lane & (CH - 1);
+            const float tdiag = sm.u.s.T[li][li];
+            const float rd = (chol && lane < kb) ? 1.0f / tdiag : 1.0f;
             float trow[CH];
 #pragma unroll
-            for (int k = 0; k < CH; ++k) trow[k] = sm.u.s.T[lane & (CH - 1)][k];
-            const float tdiag = sm.u.s.T[lane & (CH - 1)][lane & (CH - 1)];
+            for (int k = 0; k < CH; ++k) trow[k] = (k == li) ? 0.0f : sm.u.s.T[li][k] * rd;
             asm volatile("" ::: "memory");
             for (int g = w; g < nr; g += kWaves) {
                 const float v0 = sm.u.s.v[g][sm.p[min(kq + lane, NMAX - 1)]];
-                float vi = (lane < kb) ? v0 : 0.0f;
+                float vi = (lane < kb) ? v0 * rd : 0.0f;
 #pragma unroll
                 for (int k = 0; k < CH; ++k) {
-                    if (k < kb) {
-                        if (chol && lane == k) vi = vi / tdiag;
-                        const float zk = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(vi), k));
-                        if (lane > k && lane < kb) vi = fmaf(-trow[k], zk, vi);
-                    }
+                    const float zk = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(vi), k));
+                    vi = fmaf(-trow[k], zk, vi);
                 }
                 if (lane < kb) {
                     sm.u.s.z[g][lane] = vi;
@@ -815,21 +819,22 @@ struct Mixed {
                 sm.u.s.T[i][k] = v;
             }
             __syncthreads();
+            // unit-diagonal form as in the forward sweep (U and L^T have
+            // non-unit diagonals in both kinds)
+            const int li = lane & (CH - 1);
+            const float tdiag = sm.u.s.T[li][li];
+            const float rd = (lane < kb) ? 1.0f / tdiag : 1.0f;
             float trow[CH];
 #pragma unroll
-            for (int k = 0; k < CH; ++k) trow[k] = sm.u.s.T[lane & (CH - 1)][k];
-            const float tdiag = sm.u.s.T[lane & (CH - 1)][lane & (CH - 1)];
+            for (int k = 0; k < CH; ++k) trow[k] = (k == li) ? 0.0f : sm.u.s.T[li][k] * rd;
             asm volatile("" ::: "memory");
             for (int g = w; g < nr; g += kWaves) {
                 const float v0 = sm.u.s.v[g][sm.p[min(kq + lane, NMAX - 1)]];
-                float vi = (lane < kb) ? v0 : 0.0f;
+                float vi = (lane < kb) ? v0 * rd : 0.0f;
 #pragma unroll
                 for (int k = CH - 1; k >= 0; --k) {
-                    if (k < kb) {
-                        if (lane == k) vi = vi / tdiag;
-                        const float yk = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(vi), k));
-                        if (lane < k) vi = fmaf(-trow[k], yk, vi);
-                    }
+                    const float yk = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(vi), k));
+                    vi = fmaf(-trow[k], yk, vi);
                 }
                 if (lane < kb) {
                     sm.u.s.z[g][lane] = vi;

@@ -47,13 +47,72 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// (key, pos) step of the pivot search across lanes by DPP: larger key wins,
+// ties to the smaller logical position (idamax's first maximum).
+template <int CTRL>
+__device__ __forceinline__ void amax_dpp(double &key, int &kp)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(key);
+    const int lo = __builtin_amdgcn_update_dpp((int)b, (int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(b >> 32), (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    const int p2 = __builtin_amdgcn_update_dpp(kp, kp, CTRL, 0xF, 0xF, false);
+    const double k2 = __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+    const bool take = (k2 > key) | ((k2 == key) & (p2 < kp));
+    key = take ? k2 : key;
+    kp = take ? p2 : kp;
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int l)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Arg-max of (key, pos) over each group of NMAX lanes (8, 16, 32 or 64); every
+// lane of a group ends with its group's result.  Quad swaps, the half-row and
+// row mirrors reduce within a row of 16 lanes; the row leaders are combined
+// with readlane.
+template <int NMAX>
+__device__ __forceinline__ void group_argmax(double &key, int &kp, int lane)
+{
+    amax_dpp<0xB1>(key, kp);
+    amax_dpp<0x4E>(key, kp);
+    amax_dpp<0x141>(key, kp);
+    if constexpr (NMAX >= 16) amax_dpp<0x140>(key, kp);
+    if constexpr (NMAX >= 32) {
+        double k[4];
+        int p[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            k[q] = readlane_f64(key, 16 * q);
+            p[q] = __builtin_amdgcn_readlane(kp, 16 * q);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q += 2) {
+            const bool take = (k[q + 1] > k[q]) | ((k[q + 1] == k[q]) & (p[q + 1] < p[q]));
+            k[q] = take ? k[q + 1] : k[q];
+            p[q] = take ? p[q + 1] : p[q];
+        }
+        if constexpr (NMAX == 64) {
+            const bool take = (k[2] > k[0]) | ((k[2] == k[0]) & (p[2] < p[0]));
+            key = take ? k[2] : k[0];
+            kp = take ? p[2] : p[0];
+        } else {
+            key = lane < 32 ? k[0] : k[2];
+            kp = lane < 32 ? p[0] : p[2];
+        }
+    }
+}
+
 // Minimum resident waves per SIMD asked of the register allocator.
 template <int NMAX> constexpr int reg_waves_per_eu() { return NMAX >= 64 ? 2 : 4; }
 
-// Columns per unrolled chunk.  For NMAX = 64 the column loop runs over chunks
-// of 8 and the row is rotated between chunks (see rotate_left), which keeps
-// every register index a compile-time constant at 1/8 of the code size.
-template <int NMAX> constexpr int reg_chunk() { return NMAX > 32 ? 8 : NMAX; }
+// Columns per unrolled chunk.  All columns are unrolled: a rotated frame
+// (chunks of 8 with the row rotated between chunks, see rotate_left) keeps
+// the code 8 times smaller at NMAX = 64 but needs twice the registers there,
+// which spilled the row to scratch.
+template <int NMAX> constexpr int reg_chunk() { return NMAX; }
 
 // a[k] <- a[(k + CH) mod NMAX]
 template <int NMAX, int CH>
@@ -119,6 +178,26 @@ __device__ __forceinline__ void row_update(double (&a)[NMAX], int jj, double l, 
     }
 }
 
+// The same update for NMAX = 64 (one matrix per wavefront) with the operand of
+// column k taken from another lane by readlane instead of LDS: from register
+// k of lane src (FROM_ROW: the LU pivot row, not updated in this column), or
+// from value v of lane k (the Cholesky multiplier of row k).  jj is a
+// constant after the caller's column loop is unrolled.
+template <int NMAX, bool EXACT, bool NEGU_FIRST, bool FROM_ROW>
+__device__ __forceinline__ void row_update_lanes(double (&a)[NMAX], int jj, double l, int src, double v)
+{
+#pragma unroll
+    for (int k = 0; k < NMAX; ++k) {
+        if (k <= jj) continue;
+        const double uk = FROM_ROW ? readlane_f64(a[k], src) : readlane_f64(v, k);
+        double r;
+        if constexpr (NEGU_FIRST) r = fma(-uk, l, a[k]);
+        else r = fma(l, -uk, a[k]);
+        if constexpr (EXACT) a[k] = (uk != 0.0) ? r : a[k];
+        else a[k] = r;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // LU, partial pivoting.
 template <int NMAX, bool SOLVE, bool VAR>
@@ -174,22 +253,23 @@ __attribute__((amdgpu_waves_per_eu(reg_waves_per_eu<NMAX>()))) getrf_reg_kernel(
                     double key = -1.0;
                     if (cand) key = (v != v) ? ((pos == j) ? __builtin_huge_val() : -1.0) : fabs(v);
                     int kp = cand ? pos : INT_MAX;
-    #pragma unroll
-                    for (int off = NMAX / 2; off > 0; off >>= 1) {
-                        const double ok = __shfl_xor(key, off, NMAX);
-                        const int op = __shfl_xor(kp, off, NMAX);
-                        const bool take = (ok > key) || (ok == key && op < kp);
-                        key = take ? ok : key;
-                        kp = take ? op : kp;
-                    }
+                    group_argmax<NMAX>(key, kp, lane);
                     const int p = kp;
-                    if (cand && pos == p) {
+                    double piv;
+                    int ol = 0;                              // lane of the pivot row (NMAX = 64)
+                    if constexpr (NMAX == 64) {
+                        const unsigned long long ow = __builtin_amdgcn_ballot_w64(cand && pos == p);
+                        ol = ow ? (int)__builtin_ctzll(ow) : 0;
+                        piv = readlane_f64(a[jj], ol);
+                    } else {
+                        if (cand && pos == p) {
     #pragma unroll
-                        for (int k = jj; k < NMAX; ++k)
-                            if (k < kmax) buf[k] = a[k];
+                            for (int k = jj; k < NMAX; ++k)
+                                if (k < kmax) buf[k] = a[k];
+                        }
+                        wave_lds_sync();
+                        piv = buf[jj];
                     }
-                    wave_lds_sync();
-                    const double piv = buf[jj];
                     if (r == j) myipiv = p + 1;
                     pos = (pos == j) ? p : ((pos == p) ? j : pos);
                     const bool below = rowok && pos > j;
@@ -207,8 +287,13 @@ __attribute__((amdgpu_waves_per_eu(reg_waves_per_eu<NMAX>()))) getrf_reg_kernel(
                         // rank-1 update (dger).  netlib skips a column whose pivot-row
                         // entry is exactly zero; that changes the result only when a
                         // multiplier is Inf/NaN, so the exact form runs only then.
-                        if (!__builtin_isfinite(l)) row_update<NMAX, true, false>(a, jj, l, buf, kmax);
-                        else row_update<NMAX, false, false>(a, jj, l, buf, kmax);
+                        if constexpr (NMAX == 64) {
+                            if (!__builtin_isfinite(l)) row_update_lanes<NMAX, true, false, true>(a, jj, l, ol, 0.0);
+                            else row_update_lanes<NMAX, false, false, true>(a, jj, l, ol, 0.0);
+                        } else {
+                            if (!__builtin_isfinite(l)) row_update<NMAX, true, false>(a, jj, l, buf, kmax);
+                            else row_update<NMAX, false, false>(a, jj, l, buf, kmax);
+                        }
                     }
                     __builtin_amdgcn_wave_barrier();
                 }
@@ -241,14 +326,24 @@ __attribute__((amdgpu_waves_per_eu(reg_waves_per_eu<NMAX>()))) getrf_reg_kernel(
     #pragma unroll
                 for (int k = 0; k < NMAX; ++k) {
                     if (k < n) {
-                        if (pos == k && rown) {
+                        double yk[kRhsChunk];
+                        if constexpr (NMAX == 64) {
+                            const unsigned long long ow = __builtin_amdgcn_ballot_w64(pos == k && rown);
+                            const int ol = ow ? (int)__builtin_ctzll(ow) : 0;
     #pragma unroll
-                            for (int q = 0; q < kRhsChunk; ++q) xb[q] = y[q];
+                            for (int q = 0; q < kRhsChunk; ++q) yk[q] = readlane_f64(y[q], ol);
+                        } else {
+                            if (pos == k && rown) {
+    #pragma unroll
+                                for (int q = 0; q < kRhsChunk; ++q) xb[q] = y[q];
+                            }
+                            wave_lds_sync();
+    #pragma unroll
+                            for (int q = 0; q < kRhsChunk; ++q) yk[q] = xb[q];
                         }
-                        wave_lds_sync();
                         if (rown && pos > k) {
     #pragma unroll
-                            for (int q = 0; q < kRhsChunk; ++q) y[q] = fma(-xb[q], a[k], y[q]);
+                            for (int q = 0; q < kRhsChunk; ++q) y[q] = fma(-yk[q], a[k], y[q]);
                         }
                         __builtin_amdgcn_wave_barrier();
                     }
@@ -256,17 +351,32 @@ __attribute__((amdgpu_waves_per_eu(reg_waves_per_eu<NMAX>()))) getrf_reg_kernel(
     #pragma unroll
                 for (int k = NMAX - 1; k >= 0; --k) {
                     if (k < n) {
-                        if (pos == k && rown) {
+                        double yk[kRhsChunk];
+                        if constexpr (NMAX == 64) {
+                            const unsigned long long ow = __builtin_amdgcn_ballot_w64(pos == k && rown);
+                            const int ol = ow ? (int)__builtin_ctzll(ow) : 0;
+                            if (pos == k && rown) {
     #pragma unroll
-                            for (int q = 0; q < kRhsChunk; ++q) {
-                                if (y[q] != 0.0) y[q] = y[q] / a[k];
-                                xb[q] = y[q];
+                                for (int q = 0; q < kRhsChunk; ++q)
+                                    if (y[q] != 0.0) y[q] = y[q] / a[k];
                             }
+    #pragma unroll
+                            for (int q = 0; q < kRhsChunk; ++q) yk[q] = readlane_f64(y[q], ol);
+                        } else {
+                            if (pos == k && rown) {
+    #pragma unroll
+                                for (int q = 0; q < kRhsChunk; ++q) {
+                                    if (y[q] != 0.0) y[q] = y[q] / a[k];
+                                    xb[q] = y[q];
+                                }
+                            }
+                            wave_lds_sync();
+    #pragma unroll
+                            for (int q = 0; q < kRhsChunk; ++q) yk[q] = xb[q];
                         }
-                        wave_lds_sync();
                         if (rown && pos < k) {
     #pragma unroll
-                            for (int q = 0; q < kRhsChunk; ++q) y[q] = fma(-xb[q], a[k], y[q]);
+                            for (int q = 0; q < kRhsChunk; ++q) y[q] = fma(-yk[q], a[k], y[q]);
                         }
                         __builtin_amdgcn_wave_barrier();
                     }
@@ -328,9 +438,14 @@ __attribute__((amdgpu_waves_per_eu(reg_waves_per_eu<NMAX>()))) potrf_reg_kernel(
             for (int jj = 0; jj < CH; ++jj) {
                 const int j = jbase + jj;
                 if (j < n && info == 0) {
-                    if (r == j) xb[0] = a[jj];
-                    wave_lds_sync();
-                    const double ajj = xb[0];
+                    double ajj;
+                    if constexpr (NMAX == 64) {
+                        ajj = readlane_f64(a[jj], j);
+                    } else {
+                        if (r == j) xb[0] = a[jj];
+                        wave_lds_sync();
+                        ajj = xb[0];
+                    }
                     if (!(ajj > 0.0)) {
                         info = j + 1;                    // dpotrf2: a_jj <= 0 or NaN
                     } else {
@@ -339,16 +454,26 @@ __attribute__((amdgpu_waves_per_eu(reg_waves_per_eu<NMAX>()))) potrf_reg_kernel(
                         const bool below = rowok && r > j;
                         const double l = rc * a[jj];
                         if (r == j) a[jj] = ljj;
-                        if (below) {
-                            a[jj] = l;
-                            buf[r] = l;
-                        }
-                        wave_lds_sync();
-                        // trailing update a_rc -= l_rj * l_cj (dsyrk), c in (j, n);
-                        // register k holds column jbase + k, whose l_cj is buf[jbase + k]
-                        if (below) {
-                            if (!__builtin_isfinite(l)) row_update<NMAX, true, true>(a, jj, l, buf + jbase, kmax);
-                            else row_update<NMAX, false, true>(a, jj, l, buf + jbase, kmax);
+                        if constexpr (NMAX == 64) {
+                            // trailing update a_rc -= l_rj * l_cj (dsyrk), c in (j, n):
+                            // l_cj is lane c's multiplier (jbase is 0: all columns unrolled)
+                            if (below) {
+                                a[jj] = l;
+                                if (!__builtin_isfinite(l)) row_update_lanes<NMAX, true, true, false>(a, jj, l, 0, l);
+                                else row_update_lanes<NMAX, false, true, false>(a, jj, l, 0, l);
+                            }
+                        } else {
+                            if (below) {
+                                a[jj] = l;
+                                buf[r] = l;
+                            }
+                            wave_lds_sync();
+                            // trailing update a_rc -= l_rj * l_cj (dsyrk), c in (j, n);
+                            // register k holds column jbase + k, whose l_cj is buf[jbase + k]
+                            if (below) {
+                                if (!__builtin_isfinite(l)) row_update<NMAX, true, true>(a, jj, l, buf + jbase, kmax);
+                                else row_update<NMAX, false, true>(a, jj, l, buf + jbase, kmax);
+                            }
                         }
                     }
                     __builtin_amdgcn_wave_barrier();
@@ -382,17 +507,25 @@ __attribute__((amdgpu_waves_per_eu(reg_waves_per_eu<NMAX>()))) potrf_reg_kernel(
     #pragma unroll
                 for (int k = 0; k < NMAX; ++k) {
                     if (k < n) {
+                        double yk[kRhsChunk];
                         if (r == k) {
     #pragma unroll
                             for (int q = 0; q < kRhsChunk; ++q) {
                                 if (y[q] != 0.0) y[q] = y[q] / a[k];
-                                xb[q] = y[q];
+                                if constexpr (NMAX != 64) xb[q] = y[q];
                             }
                         }
-                        wave_lds_sync();
+                        if constexpr (NMAX == 64) {
+    #pragma unroll
+                            for (int q = 0; q < kRhsChunk; ++q) yk[q] = readlane_f64(y[q], k);
+                        } else {
+                            wave_lds_sync();
+    #pragma unroll
+                            for (int q = 0; q < kRhsChunk; ++q) yk[q] = xb[q];
+                        }
                         if (rowok && r > k) {
     #pragma unroll
-                            for (int q = 0; q < kRhsChunk; ++q) y[q] = fma(-xb[q], a[k], y[q]);
+                            for (int q = 0; q < kRhsChunk; ++q) y[q] = fma(-yk[q], a[k], y[q]);
                         }
                         __builtin_amdgcn_wave_barrier();
                     }

@@ -27,7 +27,7 @@ constexpr int kWaves = 4;
 __device__ __forceinline__ int pick(const int *arr, int fixed, int id) { return arr ? arr[id] : fixed; }
 
 template <bool TA, bool TB, int BT>
-__global__ void __launch_bounds__(64 * kWaves) gemm_mfma_kernel(GemmSet g, const int *prefix, int fixed_tiles)
+__global__ void __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(BT == 1 ? 4 : 3))) gemm_mfma_kernel(GemmSet g, const int *prefix, int fixed_tiles)
 {
     constexpr int TS = 16 * BT;        // tile edge: BT x BT MFMA blocks of 16x16
     const int lane = threadIdx.x & 63;
@@ -68,7 +68,53 @@ __global__ void __launch_bounds__(64 * kWaves) gemm_mfma_kernel(GemmSet g, const
 #pragma unroll
             for (int bj = 0; bj < BT; ++bj) acc[bi][bj] = d4{0.0, 0.0, 0.0, 0.0};
 
-        for (int k0 = 0; k0 < k; k0 += 4) {
+        // Main loop, 16 k per step, the next step's operands loaded while the
+        // current step runs: in slice s lane (li, lk) supplies k = k0 + 4 lk + s
+        // (a permutation of k inside the step), so a lane's four k of B (and of
+        // A^T) are one contiguous 32-byte piece.  Rows and columns past the edge
+        // load a valid address and multiply by zero.
+        const int kmain = k & ~15;
+        const int rq0 = min(row0 + li, m - 1), rq1 = min(row0 + 16 * (BT - 1) + li, m - 1);
+        const int cq0 = min(col0 + li, n - 1), cq1 = min(col0 + 16 * (BT - 1) + li, n - 1);
+        double av[2][BT][4], bv[2][BT][4];
+        auto load16 = [&](int kb, double (&ao)[BT][4], double (&bo)[BT][4]) {
+#pragma unroll
+            for (int q = 0; q < BT; ++q) {
+                const int r = q == 0 ? rq0 : rq1, c = q == 0 ? cq0 : cq1;
+                const bool rin = row0 + 16 * q + li < m, cin = col0 + 16 * q + li < n;
+#pragma unroll
+                for (int sl = 0; sl < 4; ++sl) {
+                    const int kk = kb + 4 * lk + sl;
+                    const double av_ = TA ? A[kk + (size_t)r * lda] : A[r + (size_t)kk * lda];
+                    const double bv_ = TB ? B[c + (size_t)kk * ldb] : B[kk + (size_t)c * ldb];
+                    ao[q][sl] = rin ? av_ : 0.0;
+                    bo[q][sl] = cin ? bv_ : 0.0;
+                }
+            }
+        };
+        if (kmain > 0) load16(0, av[0], bv[0]);
+#pragma unroll 1
+        for (int k0 = 0; k0 < kmain; k0 += 32) {
+            if (k0 + 16 < kmain) load16(k0 + 16, av[1], bv[1]);
+#pragma unroll
+            for (int sl = 0; sl < 4; ++sl)
+#pragma unroll
+                for (int bi = 0; bi < BT; ++bi)
+#pragma unroll
+                    for (int bj = 0; bj < BT; ++bj)
+                        acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[0][bi][sl], bv[0][bj][sl], acc[bi][bj], 0, 0, 0);
+            if (k0 + 16 >= kmain) break;
+            if (k0 + 32 < kmain) load16(k0 + 32, av[0], bv[0]);
+#pragma unroll
+            for (int sl = 0; sl < 4; ++sl)
+#pragma unroll
+                for (int bi = 0; bi < BT; ++bi)
+#pragma unroll
+                    for (int bj = 0; bj < BT; ++bj)
+                        acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[1][bi][sl], bv[1][bj][sl], acc[bi][bj], 0, 0, 0);
+        }
+        // tail, 4 k per step in order
+        for (int k0 = kmain; k0 < k; k0 += 4) {
             const int kk = k0 + lk;
             const bool kin = kk < k;
             double a[BT], b[BT];

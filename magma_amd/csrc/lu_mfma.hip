@@ -12,11 +12,13 @@
 //          thread picks the winning wavefront's row as the pivot row.  Row
 //          interchanges inside the panel are logical (pos); rows are written
 //          back to their final positions once, at the end of the panel.
-//   rows   One thread per column outside the panel: the panel's interchanges
-//          as a gather (all loads, then all stores, so nothing is read after
-//          it is overwritten), and for the columns right of the panel the
-//          unit-lower solve U12 = L11^-1 A12 in registers, L11 from LDS.
-//          -U12 is also staged in LDS for the next phase.
+//   rows   One thread per column right of the panel: the panel's
+//          interchanges as a gather (all loads, then all stores, so nothing is
+//          read after it is overwritten), and the unit-lower solve
+//          U12 = L11^-1 A12 in registers, L11 from LDS.  -U12 is also staged
+//          in LDS for the next phase.  The columns left of the panel (the L
+//          part) take all later interchanges in one pass after the last panel,
+//          a column read and written once, instead of a gather per panel.
 //   update A22 -= L21 U12 in 32x32 tiles, one tile per wavefront at a time, on
 //          v_mfma_f64_16x16x4f64.  The product is formed transposed
 //          (A22^T -= U12^T L21^T) so the C tile and the L21 operand are read
@@ -110,6 +112,8 @@ struct LuSmem {
     int src_top[PW];          // logical panel row r came from physical panel row src_top[r]
     int down_dst[PW];         // physical panel row r (< jb) moves to logical row down_dst[r] (>= jb), or -1
     int swaps[PW];
+    int pst[MT / PW][PW];     // src_top of every panel, for the deferred left-column interchanges
+    int pdd[MT / PW][PW];     // down_dst of every panel
 };
 
 template <int MT>
@@ -268,16 +272,24 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(M
             }
             if (tid < jb) ipiv[j0 + tid] = j0 + sm.swaps[tid] + 1;
             const bool exact = __syncthreads_or(bad);
+            // src_top entries come from the threads owning the moved rows, in
+            // any wavefront: read them after the barrier
+            if (tid < jb) {
+                sm.pst[j0 / PW][tid] = sm.src_top[tid];
+                sm.pdd[j0 / PW][tid] = sm.down_dst[tid];
+            }
 
             // ------------------------------------------------ rows ---------
             const int cb = j0 + jb;                 // first column right of the panel
-            const int nother = n - jb;
-            // logical top row r comes from panel row st[r]; indices past jb are
-            // clamped to valid rows (their values are discarded), so every
-            // load is unconditional and all of them are in flight together
+            // Columns right of the panel only: the columns left of it (the L
+            // part) get all later interchanges in one pass after the last
+            // panel.  Logical top row r comes from panel row st[r]; indices
+            // past jb are clamped to valid rows (their values are discarded),
+            // so every load is unconditional and all are in flight together.
+            const int nright = n - cb;
 #pragma unroll 1
-            for (int t = tid; t < nother; t += kThreads) {
-                const int c = (t < j0) ? t : t + jb;
+            for (int t = tid; t < nright; t += kThreads) {
+                const int c = cb + t;
                 double *col = A + (size_t)c * lda + j0;
                 double x[PW], wv[PW];
 #pragma unroll
@@ -391,6 +403,59 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(M
                 }
             }
             __syncthreads();
+        }
+        // Deferred interchanges of the L part: the columns of panel k get the
+        // interchanges of panels k+1, ... in order (each as the panel applied
+        // it: logical top row r from row st[r], old top row r to dd[r]),
+        // through LDS (the U12 staging buffer), with one read and one write
+        // of each column.
+        {
+            const int npan = (mn + PW - 1) / PW;
+            double *buf = &sm.Us[0][0];
+#pragma unroll 1
+            for (int k = 0; k + 1 < npan; ++k) {
+                const int c0 = PW * k, nc = min(PW, mn - c0);
+                const int r0 = PW * (k + 1), nr = m - r0;
+                __syncthreads();
+                for (int e = tid; e < nc * nr; e += kThreads) {
+                    const int cl = e / nr, r = e % nr;
+                    buf[cl * MT + r] = A[(r0 + r) + (size_t)(c0 + cl) * lda];
+                }
+#pragma unroll 1
+                for (int p = k + 1; p < npan; ++p) {
+                    const int off = PW * p - r0, jbp = min(PW, mn - PW * p);
+                    __syncthreads();
+                    constexpr int TPT = PW * PW / kThreads;
+                    double sv[TPT], tv[TPT];
+                    int base[TPT], rr[TPT], dd[TPT];     // base: column cl's row off
+#pragma unroll
+                    for (int u = 0; u < TPT; ++u) {
+                        const int e = tid + kThreads * u;
+                        const int cl = e / PW, r = e % PW;
+                        const bool on = cl < nc && r < jbp;
+                        base[u] = on ? cl * MT + off : -1;
+                        rr[u] = r;
+                        dd[u] = on ? sm.pdd[p][r] : -1;
+                        if (on) {
+                            sv[u] = buf[base[u] + sm.pst[p][r]];
+                            tv[u] = buf[base[u] + r];
+                        }
+                    }
+                    __syncthreads();
+#pragma unroll
+                    for (int u = 0; u < TPT; ++u) {
+                        if (base[u] >= 0) {
+                            buf[base[u] + rr[u]] = sv[u];
+                            if (dd[u] >= 0) buf[base[u] + dd[u]] = tv[u];
+                        }
+                    }
+                }
+                __syncthreads();
+                for (int e = tid; e < nc * nr; e += kThreads) {
+                    const int cl = e / nr, r = e % nr;
+                    A[(r0 + r) + (size_t)(c0 + cl) * lda] = buf[cl * MT + r];
+                }
+            }
         }
         if (tid == 0) s.info[id] = info;
     }

@@ -52,6 +52,25 @@ magma_int_t magma_dsposv_batched(magma_uplo_t uplo, magma_int_t n, magma_int_t n
                                  magma_int_t *dinfo_array, magma_int_t *diter_array,
                                  magma_int_t batchCount, magma_queue_t queue);
 
+/* Variable-size dsgesv / dsposv: the same per-matrix semantics and argument
+ * codes as the two above, with n, nrhs and the leading dimensions in device
+ * arrays of batchCount+1 entries (the vbatched convention of
+ * include/magma_zvbatched.h).  Orders above 512 take the FP64 path with
+ * iter = -1; orders above 2048 return MAGMA_ERR_NOT_SUPPORTED.  The checked
+ * vbatched calls read the checker's flags back, so they synchronize. */
+magma_int_t magma_dsgesv_vbatched(magma_int_t *n, magma_int_t *nrhs,
+                                  double **dA_array, magma_int_t *ldda, magma_int_t **dipiv_array,
+                                  double **dB_array, magma_int_t *lddb,
+                                  double **dX_array, magma_int_t *lddx,
+                                  magma_int_t *dinfo_array, magma_int_t *diter_array,
+                                  magma_int_t batchCount, magma_queue_t queue);
+magma_int_t magma_dsposv_vbatched(magma_uplo_t uplo, magma_int_t *n, magma_int_t *nrhs,
+                                  double **dA_array, magma_int_t *ldda,
+                                  double **dB_array, magma_int_t *lddb,
+                                  double **dX_array, magma_int_t *lddx,
+                                  magma_int_t *dinfo_array, magma_int_t *diter_array,
+                                  magma_int_t batchCount, magma_queue_t queue);
+
 /* ---- autotuner --------------------------------------------------------------
  * Several kernel families have more than one implementation; which one runs
  * for a size class comes from a table (family, size class) -> variant that

@@ -51,4 +51,9 @@ magma_int_t launch_mixed(const MixedSet &g, int max_n, hipStream_t st);
 // `lists` must be the device lists of a completed-or-enqueued launch_mixed.
 magma_int_t launch_mixed_fallback(const MixedSet &g, int max_n, int max_nrhs, hipStream_t st);
 
+// X <- B for the matrices list[0 .. *count) (device list and count).
+magma_int_t launch_copy_rhs(const MixedSet &g, const int *list, const int *count, hipStream_t st);
+// *a = 0 and *b = 0 on the stream (either may be null).
+magma_int_t launch_zero_counts(int *a, int *b, hipStream_t st);
+
 }  // namespace magma_amd

@@ -1132,6 +1132,20 @@ magma_int_t launch_mixed(const MixedSet &g, int max_n, hipStream_t st)
     return MAGMA_SUCCESS;
 }
 
+magma_int_t launch_copy_rhs(const MixedSet &g, const int *list, const int *count, hipStream_t st)
+{
+    copy_rhs_kernel<<<persistent_grid(g.count, 4), 256, 0, st>>>(g, list, count);
+    MAGMA_LAUNCH_CHECK();
+    return MAGMA_SUCCESS;
+}
+
+magma_int_t launch_zero_counts(int *a, int *b, hipStream_t st)
+{
+    zero_counts_kernel<<<1, 64, 0, st>>>(a, b);
+    MAGMA_LAUNCH_CHECK();
+    return MAGMA_SUCCESS;
+}
+
 magma_int_t launch_mixed_fallback(const MixedSet &g, int max_n, int max_nrhs, hipStream_t st)
 {
     (void)max_nrhs;

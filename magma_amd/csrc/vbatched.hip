@@ -16,6 +16,7 @@
 #include <algorithm>
 
 #include "kernels.h"
+#include "mixed.h"
 #include "workqueue.h"
 #include "tune.h"
 
@@ -42,7 +43,8 @@ __global__ void clear_kernel(int *p, int count)
 
 // Validates per-matrix sizes.  mode: 'F' factorization (m, n, lda >= max(1,m)),
 // 'C' Cholesky (n, lda >= n, reference rule), 'S' solve (n, nrhs, lda, ldb),
-// 'G' gemm (m, n, k, lda, ldb, ldc for the given transposes).
+// 'M' mixed solve (n, nrhs, lda, ldb, ldx = ldc), 'G' gemm (m, n, k, lda,
+// ldb, ldc for the given transposes).
 __global__ void check_kernel(char mode, char ta, char tb, const int *m, const int *n, const int *k,
                              const int *lda, const int *ldb, const int *ldc, int count, int limit,
                              int *flags)
@@ -60,11 +62,12 @@ __global__ void check_kernel(char mode, char ta, char tb, const int *m, const in
         if (ni < 0) flags[kFlagN] = 1;
         if (lda[i] < ni) flags[kFlagLda] = 1;
         if (ni > limit) flags[kFlagTooBig] = 1;
-    } else if (mode == 'S') {
+    } else if (mode == 'S' || mode == 'M') {
         if (ni < 0) flags[kFlagN] = 1;
         if (k[i] < 0) flags[kFlagNrhs] = 1;
         if (lda[i] < max(1, ni)) flags[kFlagLda] = 1;
         if (ldb[i] < max(1, ni)) flags[kFlagLdb] = 1;
+        if (mode == 'M' && ldc[i] < max(1, ni)) flags[kFlagLdc] = 1;
         if (ni > limit) flags[kFlagTooBig] = 1;
     } else {
         const int ki = k[i];
@@ -182,6 +185,151 @@ magma_int_t run_classes_device(char kind, MatSet s, const VScratch &v, int max_d
 }
 
 constexpr int kMaxFactorDim = 2048;
+
+// Appends ids[0 .. cnt) to a fallback list with dsgesv's iter code -1 (the
+// order is beyond the mixed kernels).
+__global__ void append_fallback_kernel(const int *ids, int cnt, int *list, int *count, int *iter)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < cnt) {
+        const int id = ids[i];
+        list[atomicAdd(count, 1)] = id;
+        if (iter) iter[id] = -1;
+    }
+}
+
+// Mixed-precision solves of matrices of different orders (kind 'L' LU, 'C'
+// Cholesky).  After the checker's read-back the host knows how many matrices
+// each size class holds:
+//   * classes up to 256 form one contiguous range of the class-sorted ids and
+//     take one persistent launch of the n <= 256 mixed kernel, the 512 class
+//     one launch of the n <= 512 kernel (larger systems first);
+//   * orders above 512 go straight to the FP64 list, with iter = -1;
+//   * the FP64 list (those plus the matrices whose refinement gave up) is
+//     binned by class on the device and every class present in the input gets
+//     its FP64 factor and solve kernels, which read their counts on the device.
+magma_int_t mixed_vbatched(char kind, int *n, int *nrhs, double **dA, int *ldda, int **dipiv, double **dB,
+                           int *lddb, double **dX, int *lddx, int *info, int *iter, int count,
+                           magma_queue_t queue, const char *name)
+{
+    hipStream_t st = stream_of(queue);
+    const size_t fl = round_up(sizeof(int) * kNumFlags, 256);
+    const size_t wq = round_up(workqueue_bytes(count), 256);
+    const size_t lb = round_up(sizeof(int) * (size_t)(count + 1), 256);
+    const size_t head = fl + wq + lb;
+    char *base = static_cast<char *>(queue_workspace(queue, head));
+    VScratch v;
+    v.host = static_cast<int *>(queue_host(queue, sizeof(int) * (kNumFlags + 2 * kNumBuckets)));
+    if (!base || !v.host) return MAGMA_ERR_DEVICE_ALLOC;
+    v.flags = reinterpret_cast<int *>(base);
+    v.q = workqueue_view(base + fl, count);
+    magma_int_t r = check_and_bucket(queue, v, 'M', 'N', 'N', nullptr, n, nrhs, ldda, lddb, lddx, count,
+                                     kMaxFactorDim, true);
+    if (r != MAGMA_SUCCESS) return r;
+    const int *f = v.host;
+    magma_int_t arg = 0;
+    if (f[kFlagN]) arg = -1;
+    else if (f[kFlagNrhs]) arg = -2;
+    else if (f[kFlagLda]) arg = -4;
+    else if (f[kFlagLdb]) arg = -7;
+    else if (f[kFlagLdc]) arg = -9;
+    if (arg != 0) {
+        magma_xerbla(name, -arg);
+        return arg;
+    }
+    if (f[kFlagTooBig]) return MAGMA_ERR_NOT_SUPPORTED;
+    int counts[kNumBuckets], offsets[kNumBuckets];
+    std::copy(v.host + kNumFlags, v.host + kNumFlags + kNumBuckets, counts);
+    std::copy(v.host + kNumFlags + kNumBuckets, v.host + kNumFlags + 2 * kNumBuckets, offsets);
+    int n_small = 0, max_small = 0, max_all = 0;
+    for (int b = 0; b < kNumBuckets; ++b) {
+        if (counts[b] == 0) continue;
+        max_all = kBucketLimit[b];
+        if (kBucketLimit[b] <= 256) {
+            n_small += counts[b];
+            max_small = kBucketLimit[b];
+        }
+    }
+    const int n_mid = counts[6];
+    const int n_large = counts[7] + counts[8];
+    // factor slots: each launch has its own grid and slot size; the launches
+    // are ordered on the stream, so they share one region
+    size_t slot_bytes = 0;
+    if (n_small) slot_bytes = sizeof(float) * mixed_factor_floats(max_small) * mixed_grid(n_small, max_small);
+    if (n_mid) slot_bytes = std::max(slot_bytes, sizeof(float) * mixed_factor_floats(512) * mixed_grid(n_mid, 512));
+    char *ws = static_cast<char *>(queue_workspace(queue, head + round_up(slot_bytes, 256)));
+    if (!ws) return MAGMA_ERR_DEVICE_ALLOC;
+    v.q = workqueue_view(ws + fl, count);
+    if (ws != base) {
+        // the workspace moved (and lost the lists): bin again; the stream order
+        // makes the new lists those the launches below read
+        r = build_workqueue(nullptr, n, count, v.q, st);
+        if (r != MAGMA_SUCCESS) return r;
+    }
+    int *fb_count = reinterpret_cast<int *>(ws + fl + wq);
+    int *fb_list = fb_count + 1;
+    r = launch_zero_counts(fb_count, nullptr, st);
+    if (r != MAGMA_SUCCESS) return r;
+    MixedSet g;
+    g.n_arr = n; g.nrhs_arr = nrhs; g.lda_arr = ldda; g.ldb_arr = lddb; g.ldx_arr = lddx;
+    g.kind = kind;
+    g.A = dA; g.B = dB; g.X = dX;
+    g.Wslot = reinterpret_cast<float *>(ws + head);
+    g.ipiv = dipiv; g.info = info; g.iter = iter;
+    if (kind == 'C') {
+        g.fbc_list = fb_list; g.fbc_count = fb_count;
+    } else {
+        g.fb_list = fb_list; g.fb_count = fb_count;
+    }
+    if (n_mid) {
+        MixedSet gm = g;
+        gm.index = v.q.ids + offsets[6];
+        gm.count = n_mid;
+        gm.slot_floats = mixed_factor_floats(512);
+        r = launch_mixed(gm, 512, st);
+        if (r != MAGMA_SUCCESS) return r;
+    }
+    if (n_small) {
+        MixedSet gs = g;
+        gs.index = v.q.ids + offsets[0];
+        gs.count = n_small;
+        gs.slot_floats = mixed_factor_floats(max_small);
+        r = launch_mixed(gs, max_small, st);
+        if (r != MAGMA_SUCCESS) return r;
+    }
+    if (n_large) {
+        append_fallback_kernel<<<ceil_div(n_large, 256), 256, 0, st>>>(v.q.ids + offsets[7], n_large, fb_list,
+                                                                       fb_count, iter);
+        MAGMA_LAUNCH_CHECK();
+    }
+    // FP64 path over the list: X <- B, then factor and solve in place, class
+    // by class (the class lists overwrite the input's, which the launches
+    // above have finished with by stream order)
+    g.count = count;
+    r = launch_copy_rhs(g, fb_list, fb_count, st);
+    if (r != MAGMA_SUCCESS) return r;
+    r = build_workqueue_list(n, fb_list, fb_count, v.q, st);
+    if (r != MAGMA_SUCCESS) return r;
+    MatSet s;
+    s.m_arr = n; s.n_arr = n; s.lda_arr = ldda;
+    s.A = dA; s.ipiv = dipiv; s.info = info;
+    s.B = dX; s.nrhs_arr = nrhs; s.ldb_arr = lddx;
+    s.count = count;
+    for (int pass = 0; pass < 2; ++pass) {
+        const char k = kind == 'C' ? (pass == 0 ? 'C' : 'l') : (pass == 0 ? 'L' : 'N');
+        for (int b = 0; b < kNumBuckets && kBucketLimit[b] <= max_all; ++b) {
+            if (counts[b] == 0) continue;
+            MatSet c = s;
+            c.index = v.q.ids;
+            c.dev_offset = v.q.offsets + b;
+            c.dev_count = v.q.counts + b;
+            c.count = counts[b];
+            r = launch_class(k, b, c, st);
+            if (r != MAGMA_SUCCESS) return r;
+        }
+    }
+    return finish(MAGMA_SUCCESS, queue);
+}
 
 }  // namespace
 }  // namespace magma_amd
@@ -562,6 +710,44 @@ void magmablas_dgemm_vbatched_nocheck(magma_trans_t transA, magma_trans_t transB
 {
     magmablas_dgemm_vbatched_max_nocheck(transA, transB, m, n, k, alpha, dA_array, ldda, dB_array, lddb,
                                          beta, dC_array, lddc, batchCount, 1 << 12, 1 << 12, 1 << 12, queue);
+}
+
+// ------------------------------------------------- mixed precision ---------
+// No counterpart in the reference (it has no batched or vbatched dsgesv); the
+// argument order and codes are those of magma_dsgesv_batched /
+// magma_dsposv_batched, with per-matrix n, nrhs and leading dimensions in
+// device arrays of batchCount+1 entries.
+magma_int_t magma_dsgesv_vbatched(magma_int_t *n, magma_int_t *nrhs, double **dA_array, magma_int_t *ldda,
+                                  magma_int_t **dipiv_array, double **dB_array, magma_int_t *lddb,
+                                  double **dX_array, magma_int_t *lddx, magma_int_t *dinfo_array,
+                                  magma_int_t *diter_array, magma_int_t batchCount, magma_queue_t queue)
+{
+    if (batchCount < 0) {
+        magma_xerbla(__func__, 12);
+        return -12;
+    }
+    if (batchCount == 0) return MAGMA_SUCCESS;
+    return mixed_vbatched('L', n, nrhs, dA_array, ldda, dipiv_array, dB_array, lddb, dX_array, lddx,
+                          dinfo_array, diter_array, batchCount, queue, __func__);
+}
+
+magma_int_t magma_dsposv_vbatched(magma_uplo_t uplo, magma_int_t *n, magma_int_t *nrhs, double **dA_array,
+                                  magma_int_t *ldda, double **dB_array, magma_int_t *lddb,
+                                  double **dX_array, magma_int_t *lddx, magma_int_t *dinfo_array,
+                                  magma_int_t *diter_array, magma_int_t batchCount, magma_queue_t queue)
+{
+    if (uplo != MagmaLower) {
+        if (uplo == MagmaUpper) std::fprintf(stderr, "%s: uplo=upper is not implemented\n", __func__);
+        magma_xerbla(__func__, 1);
+        return -1;
+    }
+    if (batchCount < 0) {
+        magma_xerbla(__func__, 12);
+        return -12;
+    }
+    if (batchCount == 0) return MAGMA_SUCCESS;
+    return mixed_vbatched('C', n, nrhs, dA_array, ldda, nullptr, dB_array, lddb, dX_array, lddx,
+                          dinfo_array, diter_array, batchCount, queue, __func__);
 }
 
 }  // extern "C"

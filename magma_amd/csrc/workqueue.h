@@ -26,6 +26,9 @@ size_t workqueue_bytes(int count);
 WorkQueue workqueue_view(void *mem, int count);
 // m may be null (square: m = n).  Enqueued on st; no host synchronization.
 magma_int_t build_workqueue(const int *m, const int *n, int count, WorkQueue q, hipStream_t st);
+// The same for the square matrices named by list[0 .. *count), where list and
+// count live on the device; ids must have room for the longest such list.
+magma_int_t build_workqueue_list(const int *n, const int *list, const int *count, WorkQueue q, hipStream_t st);
 // prefix[count+1] of ceil(m_i/tile_m)*ceil(n_i/tile_n); enqueued on st.
 magma_int_t build_tile_prefix(const int *m, const int *n, int count, int tile_m, int tile_n, int *prefix,
                               hipStream_t st);

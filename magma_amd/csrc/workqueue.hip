@@ -107,7 +107,45 @@ __global__ void __launch_bounds__(kScanThreads) tile_prefix_kernel(const int *m,
     if (tid == kScanThreads - 1) prefix[count] = part[tid];
 }
 
+// One workgroup: the ids list[0 .. *count) sorted into size classes by n[id].
+// Used for short lists built on the device (the FP64 fallbacks of the mixed
+// solvers), so one workgroup is enough.
+__global__ void __launch_bounds__(kScanThreads) bucket_list_kernel(const int *n, const int *list,
+                                                                  const int *count, WorkQueue q)
+{
+    __shared__ int cnt[kNumBuckets];
+    __shared__ int cur[kNumBuckets];
+    const int tid = threadIdx.x;
+    if (tid < kNumBuckets) cnt[tid] = 0;
+    __syncthreads();
+    const int c = *count;
+    for (int i = tid; i < c; i += kScanThreads) atomicAdd(&cnt[size_class(n[list[i]], n[list[i]])], 1);
+    __syncthreads();
+    if (tid == 0) {
+        int acc = 0;
+        for (int b = 0; b < kNumBuckets; ++b) {
+            q.counts[b] = cnt[b];
+            q.offsets[b] = acc;
+            cur[b] = acc;
+            acc += cnt[b];
+        }
+        q.offsets[kNumBuckets] = acc;
+    }
+    __syncthreads();
+    for (int i = tid; i < c; i += kScanThreads) {
+        const int id = list[i];
+        q.ids[atomicAdd(&cur[size_class(n[id], n[id])], 1)] = id;
+    }
+}
+
 }  // namespace
+
+magma_int_t build_workqueue_list(const int *n, const int *list, const int *count, WorkQueue q, hipStream_t st)
+{
+    bucket_list_kernel<<<1, kScanThreads, 0, st>>>(n, list, count, q);
+    MAGMA_LAUNCH_CHECK();
+    return MAGMA_SUCCESS;
+}
 
 size_t workqueue_bytes(int count) { return sizeof(int) * (size_t)(count + 3 * kNumBuckets + 1); }
 

@@ -106,6 +106,8 @@ PROTOTYPES = {
     # mixed precision (include/magma_amd.h)
     "magma_dsgesv_batched": (c_int, [c_int, c_int, _P, c_int, _P, _P, c_int, _P, c_int, _P, _P, c_int, _P]),
     "magma_dsposv_batched": (c_int, [c_int, c_int, c_int, _P, c_int, _P, c_int, _P, c_int, _P, _P, c_int, _P]),
+    "magma_dsgesv_vbatched": (c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, c_int, _P]),
+    "magma_dsposv_vbatched": (c_int, [c_int, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, c_int, _P]),
     # vbatched (include/magma_dvbatched.h)
     "magma_dgetrf_vbatched": (c_int, [_P, _P, _P, _P, _P, _P, c_int, _P]),
     "magma_dgetrf_vbatched_max_nocheck": (c_int, [_P, _P, c_int, _P, _P, _P, _P, c_int, _P]),

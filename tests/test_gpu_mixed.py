@@ -186,3 +186,134 @@ def test_dsgesv_argument_errors(queue):
     assert L.magma_dsposv_batched(ma.MagmaLower, 4, -1, 0, 4, 0, 4, 0, 4, 0, 0, 1, q) == -2
     assert L.magma_dsposv_batched(ma.MagmaLower, 4, 1, 0, 3, 0, 4, 0, 4, 0, 0, 1, q) == -4
     assert L.magma_dsposv_batched(ma.MagmaLower, 4, 1, 0, 4, 0, 3, 0, 4, 0, 0, 1, q) == -7
+
+
+# ---------------------------------------------------------------- vbatched ---
+def run_mixed_v(queue, mats, rhs, kind="L"):
+    """magma_dsgesv_vbatched / magma_dsposv_vbatched on matrices of different
+    orders; returns per-matrix A, X, ipiv, info, iter."""
+    import magma_amd as ma
+    count = len(mats)
+    V = ma.VarBatch(mats)
+    R = ma.VarBatch(rhs)
+    Xv = ma.VarBatch([np.zeros_like(b) for b in rhs])
+    sizes = [m.shape[0] for m in mats]
+    ipiv = ma.VarInts([max(n, 1) for n in sizes])
+    info = ma.dev_ints(np.full(count, 77))
+    it = ma.dev_ints(np.full(count, 99))
+    L = ma.load()
+    print(f"{'dsgesv' if kind == 'L' else 'dsposv'}_vbatched count={count} sizes={sizes}", flush=True)
+    if kind == "L":
+        ret = L.magma_dsgesv_vbatched(V.d_cols.data_ptr(), R.d_cols.data_ptr(), V.ptr, V.d_ld.data_ptr(),
+                                      ipiv.ptr, R.ptr, R.d_ld.data_ptr(), Xv.ptr, Xv.d_ld.data_ptr(),
+                                      info.data_ptr(), it.data_ptr(), count, queue.handle)
+    else:
+        ret = L.magma_dsposv_vbatched(ma.MagmaLower, V.d_cols.data_ptr(), R.d_cols.data_ptr(), V.ptr,
+                                      V.d_ld.data_ptr(), R.ptr, R.d_ld.data_ptr(), Xv.ptr, Xv.d_ld.data_ptr(),
+                                      info.data_ptr(), it.data_ptr(), count, queue.handle)
+    assert ret == 0
+    return (V.matrices(), Xv.matrices(), [ipiv.numpy(i, n) for i, n in enumerate(sizes)],
+            info.cpu().numpy()[:count], it.cpu().numpy()[:count])
+
+
+def _cond_matrix(seed, n):
+    from magma_amd import gen
+    if n == 0:
+        return np.zeros((0, 0))
+    if n == 1:
+        return np.full((1, 1), 0.5)
+    return gen.cond_batch(seed, 1, n)[0][0]
+
+
+def test_dsgesv_vbatched_matches_batched(queue, oracle):
+    """Each matrix of a mixed-order batch gets the bits the fixed-size call
+    gives it (same kernel, same arithmetic), including the FP64 fallbacks;
+    orders above 512 take the FP64 path with iter = -1."""
+    from magma_amd import gen
+    sizes = [256, 1, 17, 33, 64, 65, 100, 128, 129, 200, 255, 300, 512, 0, 600, 8, 9, 96, 96]
+    nrhs = [1, 2, 3, 5, 1, 2, 1, 4, 1, 1, 2, 3, 1, 1, 1, 2, 1, 1, 1]
+    mats = [_cond_matrix(20 + i, n) for i, n in enumerate(sizes)]
+    mats[17] = np.zeros((96, 96))                                     # singular: iter -3, info 1
+    U, _ = np.linalg.qr(np.random.default_rng(0).standard_normal((96, 96)))
+    W, _ = np.linalg.qr(np.random.default_rng(1).standard_normal((96, 96)))
+    mats[18] = np.ascontiguousarray(((U * 10.0 ** (-13 * np.arange(96) / 95)) @ W.T).T)   # cond 1e13
+    rhs = [gen.rhs_batch(40 + i, 1, n, r)[0] if n else np.zeros((r, 0)) for i, (n, r) in enumerate(zip(sizes, nrhs))]
+    Ag, X, ipiv, info, it = run_mixed_v(queue, mats, rhs)
+    for i, n in enumerate(sizes):
+        if n == 0:
+            assert it[i] == 0 and info[i] == 0
+            continue
+        if n > 512:
+            assert it[i] == -1 and info[i] == 0, (i, it[i])
+            assert oracle.solve_error(mats[i], X[i], rhs[i]) < 30, i
+            continue
+        if i == 17:
+            assert it[i] == -3 and info[i] == 1
+            continue
+        if i == 18:
+            assert it[i] == -11
+            LUo, ipo, _ = oracle.dgetrf(mats[i])
+            assert np.array_equal(ipiv[i], ipo) and np.array_equal(Ag[i], LUo)
+            assert np.array_equal(X[i], oracle.dgetrs(LUo, ipo, rhs[i]))
+            continue
+        Ab, Bb, Xb, ipb, infob, itb = run_mixed(queue, mats[i][None], rhs[i][None])
+        assert it[i] == itb[0] and 0 <= it[i] <= 10, (i, it[i], itb[0])
+        assert info[i] == 0
+        assert np.array_equal(X[i], Xb[0]), i
+        assert np.array_equal(ipiv[i], ipb[0]), i
+        assert np.array_equal(Ag[i], mats[i]), i                      # untouched when refinement succeeds
+        assert oracle.solve_error(mats[i], X[i], rhs[i]) <= 1.0, i
+
+
+def test_dsposv_vbatched_matches_batched(queue, oracle):
+    from magma_amd import gen
+    sizes = [1, 20, 64, 65, 100, 256, 300, 512, 0, 600, 70]
+    nrhs = [1, 3, 4, 1, 2, 1, 5, 1, 1, 2, 1]
+    mats = [gen.spd_batch(50 + i, 1, n)[0] if n else np.zeros((0, 0)) for i, n in enumerate(sizes)]
+    mats[10] = mats[10].copy()
+    mats[10][10, 10] = -5.0                                           # not SPD: FP64 path, info from potrf
+    rhs = [gen.rhs_batch(60 + i, 1, n, r)[0] if n else np.zeros((r, 0)) for i, (n, r) in enumerate(zip(sizes, nrhs))]
+    Ag, X, _, info, it = run_mixed_v(queue, mats, rhs, kind="C")
+    for i, n in enumerate(sizes):
+        if n == 0:
+            assert it[i] == 0 and info[i] == 0
+            continue
+        if n > 512:
+            assert it[i] == -1 and info[i] == 0
+            assert oracle.solve_error(mats[i], X[i], rhs[i]) < 30, i
+            continue
+        if i == 10:
+            _, info_o = oracle.dpotrf(mats[i])
+            assert it[i] < 0 and info[i] == info_o and info_o > 0
+            continue
+        Ab, Bb, Xb, _, infob, itb = run_mixed(queue, mats[i][None], rhs[i][None], kind="C")
+        assert it[i] == itb[0] and 0 <= it[i] <= 10, (i, it[i], itb[0])
+        assert info[i] == 0
+        assert np.array_equal(X[i], Xb[0]), i
+        assert oracle.solve_error(mats[i], X[i], rhs[i]) <= 1.0, i
+
+
+def test_mixed_vbatched_argument_errors(queue):
+    import magma_amd as ma
+    from magma_amd import lib as ma_lib
+    L = ma.load()
+    q = queue.handle
+    ok = ma.dev_ints([4, 4, 4])
+    neg = ma.dev_ints([4, -1, 4])
+    small = ma.dev_ints([4, 3, 4])
+    p = lambda t: t.data_ptr()
+
+    def sg(n=ok, r=ok, la=ok, lb=ok, lx=ok, count=3):
+        return L.magma_dsgesv_vbatched(p(n), p(r), 0, p(la), 0, 0, p(lb), 0, p(lx), 0, 0, count, q)
+
+    assert sg(n=neg) == -1
+    assert sg(r=neg) == -2
+    assert sg(la=small) == -4
+    assert sg(lb=small) == -7
+    assert sg(lx=small) == -9
+    assert sg(count=-1) == -12
+    assert L.magma_dsposv_vbatched(ma.MagmaUpper, p(ok), p(ok), 0, p(ok), 0, p(ok), 0, p(ok), 0, 0, 3, q) == -1
+    assert L.magma_dsposv_vbatched(ma.MagmaLower, p(neg), p(ok), 0, p(ok), 0, p(ok), 0, p(ok), 0, 0, 3, q) == -1
+    assert L.magma_dsposv_vbatched(ma.MagmaLower, p(ok), p(ok), 0, p(small), 0, p(ok), 0, p(ok), 0, 0, 3, q) == -4
+    big = ma.dev_ints([4, 3000, 4])
+    assert sg(n=big, la=big, lb=big, lx=big) == ma_lib.MAGMA_ERR_NOT_SUPPORTED

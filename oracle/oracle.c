@@ -277,6 +277,210 @@ void orc_dgemm(char transa, char transb, int m, int n, int k, double alpha,
     }
 }
 
+/* reference-BLAS dtrsm (dtrsm.f), all eight side/uplo/transpose cases:
+ *   Left:  B := alpha * inv(op(A)) * B,   Right: B := alpha * B * inv(op(A)).
+ * Loop orders, the zero skips, the alpha placement, and division (Left) versus
+ * multiplication by the reciprocal (Right) follow dtrsm.f; every update is one
+ * fma.  Reference call site: testing/testing_ztrsm_batched.cpp:234-240. */
+void orc_dtrsm(char side, char uplo, char transa, char diag, int m, int n, double alpha,
+               const double *A, int lda, double *B, int ldb)
+{
+    if (m == 0 || n == 0) return;
+    const int nounit = (diag == 'N');
+    if (alpha == 0.0) {
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < m; ++i) B_(i, j) = 0.0;
+        return;
+    }
+    if (side == 'L') {
+        if (transa == 'N') {
+            for (int j = 0; j < n; ++j) {
+                if (alpha != 1.0)
+                    for (int i = 0; i < m; ++i) B_(i, j) = alpha * B_(i, j);
+                if (uplo == 'U') {
+                    for (int k = m - 1; k >= 0; --k) {
+                        if (B_(k, j) != 0.0) {
+                            if (nounit) B_(k, j) = B_(k, j) / A_(k, k);
+                            const double bk = -B_(k, j);
+                            for (int i = 0; i < k; ++i) B_(i, j) = fma(bk, A_(i, k), B_(i, j));
+                        }
+                    }
+                } else {
+                    for (int k = 0; k < m; ++k) {
+                        if (B_(k, j) != 0.0) {
+                            if (nounit) B_(k, j) = B_(k, j) / A_(k, k);
+                            const double bk = -B_(k, j);
+                            for (int i = k + 1; i < m; ++i) B_(i, j) = fma(bk, A_(i, k), B_(i, j));
+                        }
+                    }
+                }
+            }
+        } else {
+            for (int j = 0; j < n; ++j) {
+                if (uplo == 'U') {
+                    for (int i = 0; i < m; ++i) {
+                        double t = alpha * B_(i, j);
+                        for (int k = 0; k < i; ++k) t = fma(-A_(k, i), B_(k, j), t);
+                        if (nounit) t = t / A_(i, i);
+                        B_(i, j) = t;
+                    }
+                } else {
+                    for (int i = m - 1; i >= 0; --i) {
+                        double t = alpha * B_(i, j);
+                        for (int k = i + 1; k < m; ++k) t = fma(-A_(k, i), B_(k, j), t);
+                        if (nounit) t = t / A_(i, i);
+                        B_(i, j) = t;
+                    }
+                }
+            }
+        }
+    } else if (transa == 'N') {
+        if (uplo == 'U') {
+            for (int j = 0; j < n; ++j) {
+                if (alpha != 1.0)
+                    for (int i = 0; i < m; ++i) B_(i, j) = alpha * B_(i, j);
+                for (int k = 0; k < j; ++k) {
+                    if (A_(k, j) != 0.0) {
+                        const double ak = -A_(k, j);
+                        for (int i = 0; i < m; ++i) B_(i, j) = fma(ak, B_(i, k), B_(i, j));
+                    }
+                }
+                if (nounit) {
+                    const double r = 1.0 / A_(j, j);
+                    for (int i = 0; i < m; ++i) B_(i, j) = r * B_(i, j);
+                }
+            }
+        } else {
+            for (int j = n - 1; j >= 0; --j) {
+                if (alpha != 1.0)
+                    for (int i = 0; i < m; ++i) B_(i, j) = alpha * B_(i, j);
+                for (int k = j + 1; k < n; ++k) {
+                    if (A_(k, j) != 0.0) {
+                        const double ak = -A_(k, j);
+                        for (int i = 0; i < m; ++i) B_(i, j) = fma(ak, B_(i, k), B_(i, j));
+                    }
+                }
+                if (nounit) {
+                    const double r = 1.0 / A_(j, j);
+                    for (int i = 0; i < m; ++i) B_(i, j) = r * B_(i, j);
+                }
+            }
+        }
+    } else {
+        if (uplo == 'U') {
+            for (int k = n - 1; k >= 0; --k) {
+                if (nounit) {
+                    const double r = 1.0 / A_(k, k);
+                    for (int i = 0; i < m; ++i) B_(i, k) = r * B_(i, k);
+                }
+                for (int j = 0; j < k; ++j) {
+                    if (A_(j, k) != 0.0) {
+                        const double ajk = -A_(j, k);
+                        for (int i = 0; i < m; ++i) B_(i, j) = fma(ajk, B_(i, k), B_(i, j));
+                    }
+                }
+                if (alpha != 1.0)
+                    for (int i = 0; i < m; ++i) B_(i, k) = alpha * B_(i, k);
+            }
+        } else {
+            for (int k = 0; k < n; ++k) {
+                if (nounit) {
+                    const double r = 1.0 / A_(k, k);
+                    for (int i = 0; i < m; ++i) B_(i, k) = r * B_(i, k);
+                }
+                for (int j = k + 1; j < n; ++j) {
+                    if (A_(j, k) != 0.0) {
+                        const double ajk = -A_(j, k);
+                        for (int i = 0; i < m; ++i) B_(i, j) = fma(ajk, B_(i, k), B_(i, j));
+                    }
+                }
+                if (alpha != 1.0)
+                    for (int i = 0; i < m; ++i) B_(i, k) = alpha * B_(i, k);
+            }
+        }
+    }
+}
+
+/* reference-BLAS dtrsv (dtrsv.f): x := inv(op(A)) x, increment incx >= 1.
+ * NoTrans is the column (axpy) form with the zero skip; Trans is the dot form,
+ * with i ascending for Upper and descending for Lower as in dtrsv.f (this
+ * last order differs from dtrsm's Left/Trans/Lower).  Every update is one fma.
+ * Reference call site: testing/testing_ztrsv_batched.cpp:192-196. */
+void orc_dtrsv(char uplo, char trans, char diag, int n, const double *A, int lda, double *x, int incx)
+{
+    const int nounit = (diag == 'N');
+#define X_(i) x[(size_t)(i) * (size_t)incx]
+    if (trans == 'N') {
+        if (uplo == 'U') {
+            for (int j = n - 1; j >= 0; --j) {
+                if (X_(j) != 0.0) {
+                    if (nounit) X_(j) = X_(j) / A_(j, j);
+                    const double t = -X_(j);
+                    for (int i = j - 1; i >= 0; --i) X_(i) = fma(t, A_(i, j), X_(i));
+                }
+            }
+        } else {
+            for (int j = 0; j < n; ++j) {
+                if (X_(j) != 0.0) {
+                    if (nounit) X_(j) = X_(j) / A_(j, j);
+                    const double t = -X_(j);
+                    for (int i = j + 1; i < n; ++i) X_(i) = fma(t, A_(i, j), X_(i));
+                }
+            }
+        }
+    } else {
+        if (uplo == 'U') {
+            for (int j = 0; j < n; ++j) {
+                double t = X_(j);
+                for (int i = 0; i < j; ++i) t = fma(-A_(i, j), X_(i), t);
+                if (nounit) t = t / A_(j, j);
+                X_(j) = t;
+            }
+        } else {
+            for (int j = n - 1; j >= 0; --j) {
+                double t = X_(j);
+                for (int i = n - 1; i > j; --i) t = fma(-A_(i, j), X_(i), t);
+                if (nounit) t = t / A_(j, j);
+                X_(j) = t;
+            }
+        }
+    }
+#undef X_
+}
+
+/* reference-BLAS dgemv (dgemv.f, LAPACK 3.x form without the x(j) = 0 skip):
+ * y := alpha op(A) x + beta y, increments >= 1.  beta is applied first
+ * (beta = 0 sets y to zero, so NaNs in y do not propagate); NoTrans adds
+ * (alpha x_j) a_ij column by column, Trans forms each dot product in i order
+ * and adds alpha times it.  Every update is one fma.
+ * Reference call site: testing/testing_zgemv_batched.cpp:155-159. */
+void orc_dgemv(char trans, int m, int n, double alpha, const double *A, int lda,
+               const double *x, int incx, double beta, double *y, int incy)
+{
+    const int leny = (trans == 'N') ? m : n, lenx = (trans == 'N') ? n : m;
+    if (m == 0 || n == 0 || (alpha == 0.0 && beta == 1.0)) return;
+#define X_(i) x[(size_t)(i) * (size_t)incx]
+#define Y_(i) y[(size_t)(i) * (size_t)incy]
+    if (beta != 1.0) {
+        for (int i = 0; i < leny; ++i) Y_(i) = (beta == 0.0) ? 0.0 : beta * Y_(i);
+    }
+    if (alpha == 0.0) return;
+    if (trans == 'N') {
+        for (int j = 0; j < lenx; ++j) {
+            const double t = alpha * X_(j);
+            for (int i = 0; i < m; ++i) Y_(i) = fma(t, A_(i, j), Y_(i));
+        }
+    } else {
+        for (int j = 0; j < n; ++j) {
+            double t = 0.0;
+            for (int i = 0; i < m; ++i) t = fma(A_(i, j), X_(i), t);
+            Y_(j) = fma(alpha, t, Y_(j));
+        }
+    }
+#undef X_
+#undef Y_
+}
+
 /* LAPACK dlange: 'M' max abs, 'O'/'1' one norm, 'I' infinity norm, 'F' Frobenius. */
 double orc_dlange(char norm, int m, int n, const double *A, int lda)
 {

@@ -41,6 +41,11 @@ void orc_dgetrs(char trans, int n, int nrhs, const double *A, int lda,
                 const int *ipiv, double *B, int ldb);
 void orc_dpotrf(char uplo, int n, double *A, int lda, int *info);
 void orc_dpotrs(char uplo, int n, int nrhs, const double *A, int lda, double *B, int ldb);
+void orc_dtrsm(char side, char uplo, char transa, char diag, int m, int n, double alpha,
+               const double *A, int lda, double *B, int ldb);
+void orc_dtrsv(char uplo, char trans, char diag, int n, const double *A, int lda, double *x, int incx);
+void orc_dgemv(char trans, int m, int n, double alpha, const double *A, int lda,
+               const double *x, int incx, double beta, double *y, int incy);
 void orc_dgemm(char transa, char transb, int m, int n, int k, double alpha,
                const double *A, int lda, const double *B, int ldb,
                double beta, double *C, int ldc);

@@ -30,6 +30,9 @@ _PROTO = {
     "orc_dpotrf": (None, [_C, _I, _P, _I, _P]),
     "orc_dpotrs": (None, [_C, _I, _I, _P, _I, _P, _I]),
     "orc_dgemm": (None, [_C, _C, _I, _I, _I, _D, _P, _I, _P, _I, _D, _P, _I]),
+    "orc_dtrsm": (None, [_C, _C, _C, _C, _I, _I, _D, _P, _I, _P, _I]),
+    "orc_dtrsv": (None, [_C, _C, _C, _I, _P, _I, _P, _I]),
+    "orc_dgemv": (None, [_C, _I, _I, _D, _P, _I, _P, _I, _D, _P, _I]),
     "orc_eps": (_D, []),
     "orc_dlange": (_D, [_C, _I, _I, _P, _I]),
     "orc_lu_error": (_D, [_I, _I, _P, _I, _P, _I, _P]),
@@ -120,6 +123,29 @@ def dgemm(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc):
     A = _f64(np.array(A, copy=True)); B = _f64(np.array(B, copy=True)); C = _f64(np.array(C, copy=True))
     lib().orc_dgemm(ta, tb, m, n, k, alpha, _p(A), lda, _p(B), ldb, beta, _p(C), ldc)
     return C
+
+
+def dtrsm(side, uplo, trans, diag, m, n, alpha, A, B):
+    """A: (cols, lda) storage of the triangular matrix (order m for Left, n for
+    Right); B: (n, ldb) storage of the m x n right-hand side.  Returns the
+    solution in a copy of B."""
+    A = _f64(np.array(A, copy=True)); B = _f64(np.array(B, copy=True))
+    lib().orc_dtrsm(side, uplo, trans, diag, m, n, alpha, _p(A), A.shape[1], _p(B), B.shape[1])
+    return B
+
+
+def dtrsv(uplo, trans, diag, n, A, x, incx=1):
+    """x: 1-D storage with increment incx; returns the solved copy."""
+    A = _f64(np.array(A, copy=True)); x = _f64(np.array(x, dtype=np.float64, copy=True))
+    lib().orc_dtrsv(uplo, trans, diag, n, _p(A), A.shape[1], _p(x), incx)
+    return x
+
+
+def dgemv(trans, m, n, alpha, A, x, beta, y, incx=1, incy=1):
+    A = _f64(np.array(A, copy=True)); x = _f64(np.array(x, dtype=np.float64, copy=True))
+    y = _f64(np.array(y, dtype=np.float64, copy=True))
+    lib().orc_dgemv(trans, m, n, alpha, _p(A), A.shape[1], _p(x), incx, beta, _p(y), incy)
+    return y
 
 
 def idamax(x):

@@ -512,8 +512,75 @@ def workload_serve(args, torch, ma, L, q):
                 check={}, step_ms=None, cpu=None)
 
 
+def workload_blas(args, torch, ma, L, q):
+    """magmablas_dtrsm_batched (Left, Lower, NoTrans, NonUnit, 16 right-hand
+    sides), magmablas_dtrsv_batched (Upper, NoTrans) and
+    magmablas_dgemv_batched (NoTrans) on 20,000 matrices of order 64, each
+    timed on its own.  One step runs the three calls."""
+    count, n, nrhs = 20000, 64, 16
+    rng = np.random.default_rng(5)
+    A0 = rng.uniform(-1, 1, (count, n, n)) / 8.0
+    idx = np.arange(n)
+    A0[:, idx, idx] = rng.uniform(1, 2, (count, n))
+    A0 = torch.from_numpy(A0).cuda()
+    B0 = torch.from_numpy(rng.uniform(-1, 1, (count, nrhs, n))).cuda()
+    x0 = torch.from_numpy(rng.uniform(-1, 1, (count, n))).cuda()
+    pa = ma.ptr_array(A0, count, n * n)
+    timers = {k: Timer(torch) for k in ("trsm", "trsv", "gemv")}
+
+    def make(i):
+        B = B0.clone(); x = x0.clone(); y = torch.zeros_like(x0)
+        return (B, x, y, ma.ptr_array(B, count, n * nrhs), ma.ptr_array(x, count, n), ma.ptr_array(y, count, n))
+
+    def step(c, timer):
+        B, x, y, pb, px, py = c
+        for name in ("trsm", "trsv", "gemv"):
+            e = timers[name].start() if timer else None
+            if name == "trsm":
+                L.magmablas_dtrsm_batched(ma.MagmaLeft, ma.MagmaLower, ma.MagmaNoTrans, ma.MagmaNonUnit, n, nrhs,
+                                          1.0, pa.data_ptr(), n, pb.data_ptr(), n, count, q.handle)
+            elif name == "trsv":
+                L.magmablas_dtrsv_batched(ma.MagmaUpper, ma.MagmaNoTrans, ma.MagmaNonUnit, n, pa.data_ptr(), n,
+                                          px.data_ptr(), 1, count, q.handle)
+            else:
+                L.magmablas_dgemv_batched(ma.MagmaNoTrans, n, n, 1.0, pa.data_ptr(), n, px.data_ptr(), 1, 0.0,
+                                          py.data_ptr(), 1, count, q.handle)
+            if timer:
+                timers[name].stop(e)
+
+    wall, step_ms, copies = run_fixed(args, torch, ma, L, q, make, step, count, count, "matrices")
+    check = {}
+    if args.check:
+        from oracle import pyoracle
+        A_h = A0[:16].cpu().numpy(); B_h = B0[:16].cpu().numpy(); x_h = x0[:16].cpu().numpy()
+        B_g = copies[-1][0][:16].cpu().numpy(); x_g = copies[-1][1][:16].cpu().numpy()
+        y_g = copies[-1][2][:16].cpu().numpy()
+        ok = True
+        for s in range(16):
+            ok &= np.array_equal(B_g[s], pyoracle.dtrsm(b"L", b"L", b"N", b"N", n, nrhs, 1.0, A_h[s], B_h[s]))
+            xs = pyoracle.dtrsv(b"U", b"N", b"N", n, A_h[s], x_h[s])
+            ok &= np.array_equal(x_g[s], xs)
+            ok &= np.array_equal(y_g[s], pyoracle.dgemv(b"N", n, n, 1.0, A_h[s], xs, 0.0, np.zeros(n)))
+        check = {"sample": 16, "bitwise_equal_to_oracle": bool(ok)}
+    ms = wall / args.steps * 1e3
+    kms = {k: float(np.mean(t.ms())) for k, t in timers.items()}
+    tri = 8.0 * n * (n + 1) / 2
+    byts = {"trsm": count * (tri + 16.0 * n * nrhs), "trsv": count * (tri + 16.0 * n),
+            "gemv": count * (8.0 * n * n + 8.0 * n + 8.0 * n)}
+    rates = {k: byts[k] / (kms[k] * 1e-3) / 1e9 for k in kms}
+    return dict(value=count / (ms * 1e-3), unit="matrices/s (trsm + trsv + gemv)", ms=ms, dtype="f64",
+                config={"workload": "blas_batched", "batch": count, "n": n, "nrhs": nrhs,
+                        "apis": ["magmablas_dtrsm_batched", "magmablas_dtrsv_batched", "magmablas_dgemv_batched"],
+                        "kernel_ms": kms, "algorithmic_GBps": rates,
+                        "inputs": "A U[-1,1]/8 with diagonal U[1,2] (seed 5)"},
+                roofline={"bound": "hbm", "achieved": rates["trsm"], "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                          "kernel": "trsm_kernel<L, L, N>", "kernel_ms": kms["trsm"], "bytes_per_launch": byts["trsm"]},
+                check=check, step_ms=step_ms, cpu=None)
+
+
 WORKLOADS = {
     "cond256_mixed_vs_plain": workload_cond256,
+    "blas_batched": workload_blas,
     "lu32_gesv": workload_lu32,
     "chol64_posv": workload_chol64,
     "vbatch_lu_gemm": workload_vbatch,

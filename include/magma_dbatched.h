@@ -54,6 +54,25 @@ magma_int_t magma_dposv_batched(magma_uplo_t uplo, magma_int_t n, magma_int_t nr
                                 double **dA_array, magma_int_t ldda, double **dB_array, magma_int_t lddb,
                                 magma_int_t *dinfo_array, magma_int_t batchCount, magma_queue_t queue);
 
+/* include/magma_zbatched.h:202-208 -- B := alpha inv(op(A)) B (Left) or
+ * alpha B inv(op(A)) (Right), A triangular; the operation order of the
+ * reference BLAS dtrsm, so results are those of LAPACK's CPU path bit for bit. */
+void magmablas_dtrsm_batched(magma_side_t side, magma_uplo_t uplo, magma_trans_t transA, magma_diag_t diag,
+                             magma_int_t m, magma_int_t n, double alpha,
+                             double **dA_array, magma_int_t ldda, double **dB_array, magma_int_t lddb,
+                             magma_int_t batchCount, magma_queue_t queue);
+/* include/magma_zbatched.h:237-242 -- b := inv(op(A)) b in dtrsv's operation
+ * order; n <= 4095, incb >= 1 (the reference's n / lda error codes, -5 / -8). */
+void magmablas_dtrsv_batched(magma_uplo_t uplo, magma_trans_t trans, magma_diag_t diag, magma_int_t n,
+                             double **A_array, magma_int_t lda, double **b_array, magma_int_t incb,
+                             magma_int_t batchCount, magma_queue_t queue);
+/* include/magma_zbatched.h:705-712 -- y := alpha op(A) x + beta y in dgemv's
+ * operation order; incx, incy >= 1. */
+void magmablas_dgemv_batched(magma_trans_t trans, magma_int_t m, magma_int_t n, double alpha,
+                             double **dA_array, magma_int_t ldda, double **dx_array, magma_int_t incx,
+                             double beta, double **dy_array, magma_int_t incy,
+                             magma_int_t batchCount, magma_queue_t queue);
+
 /* include/magma_zbatched.h:88-108 -- C = alpha op(A) op(B) + beta C, fixed sizes */
 void magma_dgemm_batched(magma_trans_t transA, magma_trans_t transB,
                          magma_int_t m, magma_int_t n, magma_int_t k, double alpha,

@@ -12,8 +12,8 @@ import ctypes
 import numpy as np
 
 from . import lib as _libmod
-from .lib import (MagmaConjTrans, MagmaLower, MagmaNoTrans, MagmaTrans,  # noqa: F401
-                  MagmaUpper)
+from .lib import (MagmaConjTrans, MagmaLeft, MagmaLower, MagmaNoTrans, MagmaNonUnit,  # noqa: F401
+                  MagmaRight, MagmaTrans, MagmaUnit, MagmaUpper)
 
 __all__ = ["load", "Queue", "DeviceBatch", "dgetrf_batched", "dgetrs_batched",
            "dgesv_batched", "dpotrf_batched", "dpotrs_batched", "dposv_batched"]

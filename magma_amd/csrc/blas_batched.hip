@@ -1,0 +1,416 @@
+// blas_batched.hip -- public batched triangular solves and matrix-vector
+// products: magmablas_dtrsv_batched, magmablas_dtrsm_batched and
+// magmablas_dgemv_batched (include/magma_zbatched.h:202-258,705-712).
+//
+// Every entry is computed with the operation sequence of the reference BLAS
+// routine (dtrsv.f, dtrsm.f, dgemv.f; the testers compare against them,
+// testing/testing_ztrsv_batched.cpp:192, testing/testing_ztrsm_batched.cpp:234,
+// testing/testing_zgemv_batched.cpp:155), each update one fma, so results are
+// bitwise those of the CPU restatement in the oracle.  The reference instead
+// inverts diagonal blocks and multiplies (magmablas/ztrsm_batched.cpp:168-480,
+// magmablas/ztrsv_batched.cu:70-296), which rounds differently; LAPACK's
+// order is the target here.
+//
+//   trsv  one workgroup per matrix, x in LDS.  Each of the four cases is a
+//         sequence of pivots (ascending or descending) whose final value
+//         updates every remaining entry in parallel.  For the transposed
+//         cases this "right-looking" order adds the terms of each dot product
+//         in exactly dtrsv.f's order (ascending for Upper, descending for
+//         Lower), so no case needs a serial dot.  The pivots' columns (rows,
+//         transposed) of A are staged in LDS a panel at a time with coalesced
+//         loads.
+//   trsm  one lane per independent vector (a column of B for side Left, a
+//         row for Right), each lane running dtrsm.f's loops for its vector;
+//         A's entries are the same for every lane of a matrix, so they are
+//         broadcast loads.  Two cases (Left/Trans/Lower and Right/NoTrans/
+//         Lower) add dot products in the order opposite to the order their
+//         terms become final, which rules out a parallel right-looking form
+//         for bit-exact results.
+//   gemv  NoTrans: one thread per entry of y, columns in order (coalesced A
+//         rows).  Trans: one thread per entry of y, a 64-column tile of A
+//         staged through LDS 32 rows at a time, each thread summing its
+//         column in row order.
+#include <algorithm>
+
+#include "kernels.h"
+
+namespace magma_amd {
+namespace {
+
+constexpr int kTrsvThreads = 256;
+constexpr size_t kTrsvLds = 64 * 1024;     // x plus the staged panel of A
+
+template <bool TRANS, bool UPPER>
+__global__ void __launch_bounds__(kTrsvThreads) trsv_kernel(int n, double **A_array, int lda, double **x_array,
+                                                           int incx, bool unit, int pb, int count)
+{
+    extern __shared__ double lds[];
+    double *x = lds;                 // n
+    double *pan = lds + n;           // pb x n: pan[c * n + i]
+    constexpr bool asc = (TRANS == UPPER);
+    const int tid = threadIdx.x;
+    for (int item = blockIdx.x; item < count; item += gridDim.x) {
+        const double *a = A_array[item];
+        double *xg = x_array[item];
+        for (int i = tid; i < n; i += kTrsvThreads) x[i] = xg[(size_t)i * incx];
+        for (int s0 = 0; s0 < n; s0 += pb) {
+            const int np = min(pb, n - s0);
+            __syncthreads();
+            // stage the panel: pivot p_c = step s0 + c; NoTrans column p_c,
+            // Trans row p_c, all n entries (c fastest for Trans, so each
+            // column of A is read as one contiguous run of pivots' rows)
+            if (!TRANS) {
+                for (int e = tid; e < np * n; e += kTrsvThreads) {
+                    const int c = e / n, i = e % n;
+                    const int p = asc ? s0 + c : n - 1 - (s0 + c);
+                    pan[c * n + i] = a[i + (size_t)p * lda];
+                }
+            } else {
+                for (int e = tid; e < np * n; e += kTrsvThreads) {
+                    const int c = e % np, j = e / np;
+                    const int p = asc ? s0 + c : n - 1 - (s0 + c);
+                    pan[c * n + j] = a[p + (size_t)j * lda];
+                }
+            }
+            __syncthreads();
+#pragma unroll 1
+            for (int c = 0; c < np; ++c) {
+                const int p = asc ? s0 + c : n - 1 - (s0 + c);
+                const double *col = pan + c * n;
+                const double xr = x[p];
+                bool skip = false;
+                double xp;
+                if (!TRANS) {
+                    // dtrsv.f NoTrans: the zero skip covers the division
+                    skip = (xr == 0.0);
+                    xp = (skip || unit) ? xr : xr / col[p];
+                } else {
+                    xp = unit ? xr : xr / col[p];
+                }
+                const int lo = asc ? p + 1 : 0, hi = asc ? n : p;
+                if (!skip)
+                    for (int i = lo + tid; i < hi; i += kTrsvThreads) x[i] = fma(-xp, col[i], x[i]);
+                __syncthreads();
+                if (tid == 0) x[p] = xp;
+            }
+        }
+        __syncthreads();
+        for (int i = tid; i < n; i += kTrsvThreads) xg[(size_t)i * incx] = x[i];
+        __syncthreads();
+    }
+}
+
+// One lane per vector: Left -> column v of B (length m), Right -> row v
+// (length n).  The loops are dtrsm.f's, restricted to that vector.
+template <char SIDE, char UPLO, char TRANS>
+__global__ void __launch_bounds__(64) trsm_kernel(int m, int n, double alpha, double **A_array, int lda,
+                                                  double **B_array, int ldb, bool unit, int count)
+{
+    const int nv = SIDE == 'L' ? n : m;
+    const int vblocks = (nv + 63) / 64;
+    for (int item = blockIdx.x; item < count * vblocks; item += gridDim.x) {
+        const int id = item / vblocks;
+        const int v = (item % vblocks) * 64 + (int)threadIdx.x;
+        if (v >= nv) continue;
+        const double *a = A_array[id];
+        double *b = B_array[id];
+#define AA(i, j) a[(size_t)(i) + (size_t)(j) * lda]
+#define XV(k) (SIDE == 'L' ? b[(size_t)(k) + (size_t)v * ldb] : b[(size_t)v + (size_t)(k) * ldb])
+        if (alpha == 0.0) {
+            for (int k = 0; k < (SIDE == 'L' ? m : n); ++k) XV(k) = 0.0;
+            continue;
+        }
+        if (SIDE == 'L') {
+            if (TRANS == 'N') {
+                if (alpha != 1.0)
+                    for (int i = 0; i < m; ++i) XV(i) = alpha * XV(i);
+                if (UPLO == 'U') {
+                    for (int k = m - 1; k >= 0; --k) {
+                        double bk = XV(k);
+                        if (bk != 0.0) {
+                            if (!unit) bk = bk / AA(k, k);
+                            XV(k) = bk;
+                            for (int i = 0; i < k; ++i) XV(i) = fma(-bk, AA(i, k), XV(i));
+                        }
+                    }
+                } else {
+                    for (int k = 0; k < m; ++k) {
+                        double bk = XV(k);
+                        if (bk != 0.0) {
+                            if (!unit) bk = bk / AA(k, k);
+                            XV(k) = bk;
+                            for (int i = k + 1; i < m; ++i) XV(i) = fma(-bk, AA(i, k), XV(i));
+                        }
+                    }
+                }
+            } else if (UPLO == 'U') {
+                for (int i = 0; i < m; ++i) {
+                    double t = alpha * XV(i);
+                    for (int k = 0; k < i; ++k) t = fma(-AA(k, i), XV(k), t);
+                    if (!unit) t = t / AA(i, i);
+                    XV(i) = t;
+                }
+            } else {
+                for (int i = m - 1; i >= 0; --i) {
+                    double t = alpha * XV(i);
+                    for (int k = i + 1; k < m; ++k) t = fma(-AA(k, i), XV(k), t);
+                    if (!unit) t = t / AA(i, i);
+                    XV(i) = t;
+                }
+            }
+        } else if (TRANS == 'N') {
+            if (UPLO == 'U') {
+                for (int j = 0; j < n; ++j) {
+                    double t = XV(j);
+                    if (alpha != 1.0) t = alpha * t;
+                    for (int k = 0; k < j; ++k) {
+                        const double akj = AA(k, j);
+                        if (akj != 0.0) t = fma(-akj, XV(k), t);
+                    }
+                    if (!unit) t = (1.0 / AA(j, j)) * t;
+                    XV(j) = t;
+                }
+            } else {
+                for (int j = n - 1; j >= 0; --j) {
+                    double t = XV(j);
+                    if (alpha != 1.0) t = alpha * t;
+                    for (int k = j + 1; k < n; ++k) {
+                        const double akj = AA(k, j);
+                        if (akj != 0.0) t = fma(-akj, XV(k), t);
+                    }
+                    if (!unit) t = (1.0 / AA(j, j)) * t;
+                    XV(j) = t;
+                }
+            }
+        } else if (UPLO == 'U') {
+            for (int k = n - 1; k >= 0; --k) {
+                double bk = XV(k);
+                if (!unit) bk = (1.0 / AA(k, k)) * bk;
+                for (int j = 0; j < k; ++j) {
+                    const double ajk = AA(j, k);
+                    if (ajk != 0.0) XV(j) = fma(-ajk, bk, XV(j));
+                }
+                if (alpha != 1.0) bk = alpha * bk;
+                XV(k) = bk;
+            }
+        } else {
+            for (int k = 0; k < n; ++k) {
+                double bk = XV(k);
+                if (!unit) bk = (1.0 / AA(k, k)) * bk;
+                for (int j = k + 1; j < n; ++j) {
+                    const double ajk = AA(j, k);
+                    if (ajk != 0.0) XV(j) = fma(-ajk, bk, XV(j));
+                }
+                if (alpha != 1.0) bk = alpha * bk;
+                XV(k) = bk;
+            }
+        }
+#undef AA
+#undef XV
+    }
+}
+
+// y := alpha A x + beta y, one thread per row.
+__global__ void __launch_bounds__(256) gemv_n_kernel(int m, int n, double alpha, double **A_array, int lda,
+                                                    double **x_array, int incx, double beta, double **y_array,
+                                                    int incy, int count)
+{
+    const int rblocks = (m + 255) / 256;
+    for (int item = blockIdx.x; item < count * rblocks; item += gridDim.x) {
+        const int id = item / rblocks;
+        const int i = (item % rblocks) * 256 + (int)threadIdx.x;
+        if (i >= m) continue;
+        const double *a = A_array[id] + i;
+        const double *x = x_array[id];
+        double *yp = y_array[id] + (size_t)i * incy;
+        double y = (beta == 0.0) ? 0.0 : (beta == 1.0 ? *yp : beta * *yp);
+        if (alpha != 0.0) {
+            for (int j = 0; j < n; ++j) y = fma(alpha * x[(size_t)j * incx], a[(size_t)j * lda], y);
+        }
+        *yp = y;
+    }
+}
+
+// y := alpha A' x + beta y, one thread per column, 64 columns per workgroup,
+// A staged 32 rows at a time.
+__global__ void __launch_bounds__(64) gemv_t_kernel(int m, int n, double alpha, double **A_array, int lda,
+                                                   double **x_array, int incx, double beta, double **y_array,
+                                                   int incy, int count)
+{
+    __shared__ double tile[64][33];
+    const int lane = threadIdx.x;
+    const int cblocks = (n + 63) / 64;
+    for (int item = blockIdx.x; item < count * cblocks; item += gridDim.x) {
+        const int id = item / cblocks;
+        const int j0 = (item % cblocks) * 64;
+        const double *a = A_array[id];
+        const double *x = x_array[id];
+        const int j = j0 + lane;
+        double t = 0.0;
+        if (alpha != 0.0) {
+            for (int i0 = 0; i0 < m; i0 += 32) {
+                const int rows = min(32, m - i0);
+                // 64 columns x 32 rows; lanes 0-31 read one column, 32-63 the next
+#pragma unroll 4
+                for (int k = 0; k < 32; ++k) {
+                    const int c = 2 * k + (lane >> 5), r = lane & 31;
+                    if (r < rows && j0 + c < n) tile[c][r] = a[(size_t)(i0 + r) + (size_t)(j0 + c) * lda];
+                }
+                __syncthreads();
+                if (j < n)
+                    for (int r = 0; r < rows; ++r) t = fma(tile[lane][r], x[(size_t)(i0 + r) * incx], t);
+                __syncthreads();
+            }
+        }
+        if (j < n) {
+            double *yp = y_array[id] + (size_t)j * incy;
+            double y = (beta == 0.0) ? 0.0 : (beta == 1.0 ? *yp : beta * *yp);
+            if (alpha != 0.0) y = fma(alpha, t, y);
+            *yp = y;
+        }
+    }
+}
+
+magma_int_t sync_unless_async(magma_queue_t queue)
+{
+    if (!(queue && queue->async)) MAGMA_HIP_CHECK(hipStreamSynchronize(stream_of(queue)));
+    return MAGMA_SUCCESS;
+}
+
+template <bool T, bool U>
+magma_int_t launch_trsv(int n, double **A, int lda, double **x, int incx, bool unit, int count, hipStream_t st)
+{
+    int pb = (int)(kTrsvLds / (sizeof(double) * (size_t)n)) - 1;
+    pb = std::max(1, std::min(32, pb));
+    const size_t lds = sizeof(double) * (size_t)n * (pb + 1);
+    trsv_kernel<T, U><<<persistent_grid(count, 2), kTrsvThreads, lds, st>>>(n, A, lda, x, incx, unit, pb, count);
+    MAGMA_LAUNCH_CHECK();
+    return MAGMA_SUCCESS;
+}
+
+template <char S, char U, char T>
+magma_int_t launch_trsm(int m, int n, double alpha, double **A, int lda, double **B, int ldb, bool unit, int count,
+                        hipStream_t st)
+{
+    const long items = (long)count * (((S == 'L' ? n : m) + 63) / 64);
+    trsm_kernel<S, U, T><<<persistent_grid(items, 8), 64, 0, st>>>(m, n, alpha, A, lda, B, ldb, unit, count);
+    MAGMA_LAUNCH_CHECK();
+    return MAGMA_SUCCESS;
+}
+
+}  // namespace
+}  // namespace magma_amd
+
+using namespace magma_amd;
+
+extern "C" {
+
+// Largest order the batched trsv takes (x and one pivot of A in 64 KB of LDS).
+#define MAGMA_AMD_TRSV_MAX_N 4095
+
+// include/magma_zbatched.h:237-242.  Argument checks and codes of
+// magmablas/ztrsv_batched.cu:82-92, which report n as -5 and lda as -8;
+// incb must be >= 1 (reported on stderr; the reference does not check it).
+void magmablas_dtrsv_batched(magma_uplo_t uplo, magma_trans_t trans, magma_diag_t diag, magma_int_t n,
+                             double **A_array, magma_int_t lda, double **b_array, magma_int_t incb,
+                             magma_int_t batchCount, magma_queue_t queue)
+{
+    magma_int_t info = 0;
+    if (uplo != MagmaUpper && uplo != MagmaLower) info = -1;
+    else if (trans != MagmaNoTrans && trans != MagmaTrans && trans != MagmaConjTrans) info = -2;
+    else if (diag != MagmaUnit && diag != MagmaNonUnit) info = -3;
+    else if (n < 0) info = -5;
+    else if (lda < std::max(1, n)) info = -8;
+    if (info != 0) {
+        magma_xerbla(__func__, -info);
+        return;
+    }
+    if (n == 0 || batchCount <= 0) return;
+    if (incb < 1) {
+        std::fprintf(stderr, "%s: incb = %d is not supported (must be >= 1)\n", __func__, incb);
+        return;
+    }
+    if (n > MAGMA_AMD_TRSV_MAX_N) {
+        std::fprintf(stderr, "%s: n = %d exceeds the supported %d\n", __func__, n, MAGMA_AMD_TRSV_MAX_N);
+        return;
+    }
+    hipStream_t st = stream_of(queue);
+    const bool t = trans != MagmaNoTrans, u = uplo == MagmaUpper, unit = diag == MagmaUnit;
+    magma_int_t r = t ? (u ? launch_trsv<true, true>(n, A_array, lda, b_array, incb, unit, batchCount, st)
+                           : launch_trsv<true, false>(n, A_array, lda, b_array, incb, unit, batchCount, st))
+                      : (u ? launch_trsv<false, true>(n, A_array, lda, b_array, incb, unit, batchCount, st)
+                           : launch_trsv<false, false>(n, A_array, lda, b_array, incb, unit, batchCount, st));
+    if (r == MAGMA_SUCCESS) (void)sync_unless_async(queue);
+}
+
+// include/magma_zbatched.h:202-208; argument checks of
+// magmablas/ztrsm_batched.cpp:811-829.
+void magmablas_dtrsm_batched(magma_side_t side, magma_uplo_t uplo, magma_trans_t transA, magma_diag_t diag,
+                             magma_int_t m, magma_int_t n, double alpha, double **dA_array, magma_int_t ldda,
+                             double **dB_array, magma_int_t lddb, magma_int_t batchCount, magma_queue_t queue)
+{
+    const magma_int_t nrowA = side == MagmaLeft ? m : n;
+    magma_int_t info = 0;
+    if (side != MagmaLeft && side != MagmaRight) info = -1;
+    else if (uplo != MagmaUpper && uplo != MagmaLower) info = -2;
+    else if (transA != MagmaNoTrans && transA != MagmaTrans && transA != MagmaConjTrans) info = -3;
+    else if (diag != MagmaUnit && diag != MagmaNonUnit) info = -4;
+    else if (m < 0) info = -5;
+    else if (n < 0) info = -6;
+    else if (ldda < std::max(1, nrowA)) info = -9;
+    else if (lddb < std::max(1, m)) info = -11;
+    else if (batchCount < 0) info = -12;
+    if (info != 0) {
+        magma_xerbla(__func__, -info);
+        return;
+    }
+    if (m == 0 || n == 0 || batchCount == 0) return;
+    hipStream_t st = stream_of(queue);
+    const bool unit = diag == MagmaUnit;
+    const bool l = side == MagmaLeft, u = uplo == MagmaUpper, t = transA != MagmaNoTrans;
+    magma_int_t r;
+#define TRSM(S, U, T) launch_trsm<S, U, T>(m, n, alpha, dA_array, ldda, dB_array, lddb, unit, batchCount, st)
+    if (l) r = u ? (t ? TRSM('L', 'U', 'T') : TRSM('L', 'U', 'N')) : (t ? TRSM('L', 'L', 'T') : TRSM('L', 'L', 'N'));
+    else   r = u ? (t ? TRSM('R', 'U', 'T') : TRSM('R', 'U', 'N')) : (t ? TRSM('R', 'L', 'T') : TRSM('R', 'L', 'N'));
+#undef TRSM
+    if (r == MAGMA_SUCCESS) (void)sync_unless_async(queue);
+}
+
+// include/magma_zbatched.h:705-712; argument checks of
+// magmablas/zgemv_batched_core.cu:112-128.
+void magmablas_dgemv_batched(magma_trans_t trans, magma_int_t m, magma_int_t n, double alpha,
+                             double **dA_array, magma_int_t ldda, double **dx_array, magma_int_t incx,
+                             double beta, double **dy_array, magma_int_t incy, magma_int_t batchCount,
+                             magma_queue_t queue)
+{
+    magma_int_t info = 0;
+    if (trans != MagmaNoTrans && trans != MagmaTrans && trans != MagmaConjTrans) info = -1;
+    else if (m < 0) info = -2;
+    else if (n < 0) info = -3;
+    else if (ldda < std::max(1, m)) info = -6;
+    else if (incx < 1) info = -8;
+    else if (incy < 1) info = -11;
+    else if (batchCount < 0) info = -12;
+    if (info != 0) {
+        magma_xerbla(__func__, -info);
+        return;
+    }
+    if (m == 0 || n == 0 || batchCount == 0 || (alpha == 0.0 && beta == 1.0)) return;
+    hipStream_t st = stream_of(queue);
+    if (trans == MagmaNoTrans) {
+        const long items = (long)batchCount * ((m + 255) / 256);
+        gemv_n_kernel<<<persistent_grid(items, 4), 256, 0, st>>>(m, n, alpha, dA_array, ldda, dx_array, incx, beta,
+                                                                 dy_array, incy, batchCount);
+    } else {
+        const long items = (long)batchCount * ((n + 63) / 64);
+        gemv_t_kernel<<<persistent_grid(items, 8), 64, 0, st>>>(m, n, alpha, dA_array, ldda, dx_array, incx, beta,
+                                                                dy_array, incy, batchCount);
+    }
+    if (hipGetLastError() != hipSuccess) {
+        std::fprintf(stderr, "%s: kernel launch failed\n", __func__);
+        return;
+    }
+    (void)sync_unless_async(queue);
+}
+
+}  // extern "C"

@@ -103,6 +103,11 @@ PROTOTYPES = {
                                    c_double, _P, c_int, c_int, _P]),
     "magmablas_dgemm_batched": (None, [c_int, c_int, c_int, c_int, c_int, c_double, _P, c_int, _P, c_int,
                                        c_double, _P, c_int, c_int, _P]),
+    "magmablas_dtrsm_batched": (None, [c_int, c_int, c_int, c_int, c_int, c_int, c_double, _P, c_int, _P, c_int,
+                                       c_int, _P]),
+    "magmablas_dtrsv_batched": (None, [c_int, c_int, c_int, c_int, _P, c_int, _P, c_int, c_int, _P]),
+    "magmablas_dgemv_batched": (None, [c_int, c_int, c_int, c_double, _P, c_int, _P, c_int, c_double, _P, c_int,
+                                       c_int, _P]),
     # mixed precision (include/magma_amd.h)
     "magma_dsgesv_batched": (c_int, [c_int, c_int, _P, c_int, _P, _P, c_int, _P, c_int, _P, _P, c_int, _P]),
     "magma_dsposv_batched": (c_int, [c_int, c_int, c_int, _P, c_int, _P, c_int, _P, c_int, _P, _P, c_int, _P]),

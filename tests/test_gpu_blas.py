@@ -1,0 +1,143 @@
+"""GPU parity: magmablas_dtrsm_batched, magmablas_dtrsv_batched and
+magmablas_dgemv_batched against the oracle's restatements of the reference
+BLAS (dtrsm.f, dtrsv.f, dgemv.f).
+
+Bar: bitwise equal to the oracle for every case, side, triangle, transpose,
+diagonal, alpha / beta (including 0 and 1) and increment; the triangle that
+must not be read (and the diagonal for diag = Unit) holds NaN, and exact zeros
+in the right-hand sides exercise dtrsm's / dtrsv's zero skips.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def same(a, b):
+    a = np.asarray(a); b = np.asarray(b)
+    return a.shape == b.shape and bool(np.all((a == b) | (np.isnan(a) & np.isnan(b))))
+
+
+def tri_batch(rng, count, k, uplo, diag):
+    """(count, k, k) storage of triangular matrices with a dominant diagonal
+    and NaN where the routine must not read."""
+    out = np.empty((count, k, k))
+    for s in range(count):
+        M = rng.uniform(-1, 1, (k, k)) / max(k, 1) ** 0.5
+        np.fill_diagonal(M, rng.uniform(1, 2, k) * rng.choice([-1, 1], k))
+        keep = np.triu(np.ones((k, k), bool)) if uplo == "U" else np.tril(np.ones((k, k), bool))
+        if diag == "U":
+            np.fill_diagonal(keep, False)
+        M = np.where(keep, M, np.nan)
+        out[s] = M.T                                   # storage[j, i] = A(i, j)
+    return out
+
+
+def rhs(rng, count, cols, rows, zeros=True):
+    B = rng.uniform(-1, 1, (count, cols, rows))
+    if zeros:
+        B[rng.uniform(size=B.shape) < 0.1] = 0.0
+    return B
+
+
+def enums(side, uplo, trans, diag):
+    import magma_amd as ma
+    return ({"L": ma.MagmaLeft, "R": ma.MagmaRight}[side] if side else None,
+            ma.MagmaUpper if uplo == "U" else ma.MagmaLower,
+            ma.MagmaNoTrans if trans == "N" else ma.MagmaTrans,
+            ma.MagmaNonUnit if diag == "N" else ma.MagmaUnit)
+
+
+TRSM_CASES = list(itertools.product("LR", "UL", "NT", "NU"))
+
+
+@pytest.mark.parametrize("side,uplo,trans,diag", TRSM_CASES)
+def test_trsm_batched_bitexact(queue, oracle, side, uplo, trans, diag):
+    import magma_amd as ma
+    L = ma.load()
+    rng = np.random.default_rng(TRSM_CASES.index((side, uplo, trans, diag)))
+    es, eu, et, ed = enums(side, uplo, trans, diag)
+    for (m, n), alpha in itertools.product([(1, 1), (7, 5), (33, 17), (64, 70), (100, 3)], [1.0, 0.7, 0.0]):
+        count = 5
+        k = m if side == "L" else n
+        A = tri_batch(rng, count, k, uplo, diag)
+        B = rhs(rng, count, n, m)
+        dA = ma.DeviceBatch(A, ld=k + 3)
+        dB = ma.DeviceBatch(B, ld=m + 1)
+        print(f"trsm {side}{uplo}{trans}{diag} m={m} n={n} alpha={alpha}", flush=True)
+        L.magmablas_dtrsm_batched(es, eu, et, ed, m, n, alpha, dA.ptr, dA.ld, dB.ptr, dB.ld, count, queue.handle)
+        got = dB.numpy()
+        for s in range(count):
+            want = oracle.dtrsm(side.encode(), uplo.encode(), trans.encode(), diag.encode(), m, n, alpha, A[s], B[s])
+            assert same(got[s], want), (m, n, alpha, s)
+
+
+@pytest.mark.parametrize("uplo,trans,diag", list(itertools.product("UL", "NT", "NU")))
+def test_trsv_batched_bitexact(queue, oracle, uplo, trans, diag):
+    import magma_amd as ma
+    L = ma.load()
+    rng = np.random.default_rng(40)
+    _, eu, et, ed = enums(None, uplo, trans, diag)
+    for n, incb in itertools.product([1, 5, 64, 100, 300, 1100], [1, 2]):
+        count = 4
+        A = tri_batch(rng, count, n, uplo, diag)
+        X = np.full((count, 1, n * incb), 9.0)
+        X[:, 0, ::incb] = rhs(rng, count, 1, n)[:, 0, :]
+        dA = ma.DeviceBatch(A, ld=n + 2)
+        dX = ma.DeviceBatch(X)
+        print(f"trsv {uplo}{trans}{diag} n={n} incb={incb}", flush=True)
+        L.magmablas_dtrsv_batched(eu, et, ed, n, dA.ptr, dA.ld, dX.ptr, incb, count, queue.handle)
+        got = dX.numpy()
+        for s in range(count):
+            want = oracle.dtrsv(uplo.encode(), trans.encode(), diag.encode(), n, A[s], X[s, 0], incb)
+            assert same(got[s, 0], want), (n, incb, s)
+
+
+@pytest.mark.parametrize("trans", "NT")
+def test_gemv_batched_bitexact(queue, oracle, trans):
+    import magma_amd as ma
+    L = ma.load()
+    rng = np.random.default_rng(41)
+    et = ma.MagmaNoTrans if trans == "N" else ma.MagmaTrans
+    sizes = [(1, 1), (33, 17), (17, 33), (300, 70), (64, 64), (5, 130)]
+    for (m, n), (alpha, beta), (incx, incy) in itertools.product(
+            sizes, [(1.3, -0.4), (1.0, 0.0), (0.0, 2.0), (-0.5, 1.0)], [(1, 1), (2, 3)]):
+        count = 4
+        lx, ly = (n, m) if trans == "N" else (m, n)
+        A = rng.uniform(-1, 1, (count, n, m))
+        X = np.full((count, 1, lx * incx), 7.0)
+        X[:, 0, ::incx] = rng.uniform(-1, 1, (count, lx))
+        Y = np.full((count, 1, ly * incy), 7.0)
+        Y[:, 0, ::incy] = rng.uniform(-1, 1, (count, ly))
+        if beta == 0.0:
+            Y[:, 0, ::incy] = np.nan                     # beta = 0: y is not read
+        dA = ma.DeviceBatch(A, ld=m + 1)
+        dX = ma.DeviceBatch(X)
+        dY = ma.DeviceBatch(Y)
+        print(f"gemv {trans} m={m} n={n} alpha={alpha} beta={beta} inc=({incx},{incy})", flush=True)
+        L.magmablas_dgemv_batched(et, m, n, alpha, dA.ptr, dA.ld, dX.ptr, incx, beta, dY.ptr, incy, count,
+                                  queue.handle)
+        got = dY.numpy()
+        for s in range(count):
+            want = oracle.dgemv(trans.encode(), m, n, alpha, A[s], X[s, 0], beta, Y[s, 0], incx, incy)
+            assert same(got[s, 0], want), (m, n, alpha, beta, incx, incy, s)
+
+
+def test_blas_batched_argument_errors_leave_data(queue):
+    """Invalid arguments are reported (magma_xerbla) and nothing is written."""
+    import magma_amd as ma
+    L = ma.load()
+    A = np.eye(4)[None].repeat(2, 0)
+    B = np.ones((2, 3, 4))
+    dA = ma.DeviceBatch(A)
+    dB = ma.DeviceBatch(B)
+    q = queue.handle
+    L.magmablas_dtrsm_batched(0, ma.MagmaUpper, ma.MagmaNoTrans, ma.MagmaNonUnit, 4, 3, 2.0, dA.ptr, 4, dB.ptr, 4, 2, q)
+    L.magmablas_dtrsm_batched(ma.MagmaLeft, ma.MagmaUpper, ma.MagmaNoTrans, ma.MagmaNonUnit, 4, 3, 2.0, dA.ptr, 3,
+                              dB.ptr, 4, 2, q)
+    L.magmablas_dtrsv_batched(ma.MagmaUpper, ma.MagmaNoTrans, ma.MagmaNonUnit, 4, dA.ptr, 4, dB.ptr, 0, 2, q)
+    L.magmablas_dgemv_batched(ma.MagmaNoTrans, 4, 4, 2.0, dA.ptr, 4, dB.ptr, 0, 0.0, dB.ptr, 1, 2, q)
+    queue.sync()
+    assert np.array_equal(dB.numpy(), B)

@@ -20,9 +20,10 @@
 //         transposed) of A are staged in LDS a panel at a time with coalesced
 //         loads.
 //   trsm  one lane per independent vector (a column of B for side Left, a
-//         row for Right), each lane running dtrsm.f's loops for its vector;
-//         A's entries are the same for every lane of a matrix, so they are
-//         broadcast loads.  Two cases (Left/Trans/Lower and Right/NoTrans/
+//         row for Right), each lane running dtrsm.f's loops for its vector,
+//         held in LDS when it is at most 128 long; matrices with fewer than
+//         64 vectors share a wavefront.  A's entries are the same for every
+//         lane of a matrix, so they are broadcast loads.  Two cases (Left/Trans/Lower and Right/NoTrans/
 //         Lower) add dot products in the order opposite to the order their
 //         terms become final, which rules out a parallel right-looking form
 //         for bit-exact results.
@@ -38,11 +39,14 @@ namespace magma_amd {
 namespace {
 
 constexpr int kTrsvThreads = 256;
+constexpr int kTrsmStagedMax = 128;        // vectors up to this length live in LDS (64 KB per wavefront)
 constexpr size_t kTrsvLds = 64 * 1024;     // x plus the staged panel of A
 
-template <bool TRANS, bool UPPER>
-__global__ void __launch_bounds__(kTrsvThreads) trsv_kernel(int n, double **A_array, int lda, double **x_array,
-                                                           int incx, bool unit, int pb, int count)
+// NT threads per matrix: one wavefront for short vectors (its barriers are
+// nearly free), four otherwise.
+template <bool TRANS, bool UPPER, int NT>
+__global__ void __launch_bounds__(NT) trsv_kernel(int n, double **A_array, int lda, double **x_array,
+                                                  int incx, bool unit, int pb, int count)
 {
     extern __shared__ double lds[];
     double *x = lds;                 // n
@@ -52,7 +56,7 @@ __global__ void __launch_bounds__(kTrsvThreads) trsv_kernel(int n, double **A_ar
     for (int item = blockIdx.x; item < count; item += gridDim.x) {
         const double *a = A_array[item];
         double *xg = x_array[item];
-        for (int i = tid; i < n; i += kTrsvThreads) x[i] = xg[(size_t)i * incx];
+        for (int i = tid; i < n; i += NT) x[i] = xg[(size_t)i * incx];
         for (int s0 = 0; s0 < n; s0 += pb) {
             const int np = min(pb, n - s0);
             __syncthreads();
@@ -60,13 +64,13 @@ __global__ void __launch_bounds__(kTrsvThreads) trsv_kernel(int n, double **A_ar
             // Trans row p_c, all n entries (c fastest for Trans, so each
             // column of A is read as one contiguous run of pivots' rows)
             if (!TRANS) {
-                for (int e = tid; e < np * n; e += kTrsvThreads) {
+                for (int e = tid; e < np * n; e += NT) {
                     const int c = e / n, i = e % n;
                     const int p = asc ? s0 + c : n - 1 - (s0 + c);
                     pan[c * n + i] = a[i + (size_t)p * lda];
                 }
             } else {
-                for (int e = tid; e < np * n; e += kTrsvThreads) {
+                for (int e = tid; e < np * n; e += NT) {
                     const int c = e % np, j = e / np;
                     const int p = asc ? s0 + c : n - 1 - (s0 + c);
                     pan[c * n + j] = a[p + (size_t)j * lda];
@@ -89,37 +93,48 @@ __global__ void __launch_bounds__(kTrsvThreads) trsv_kernel(int n, double **A_ar
                 }
                 const int lo = asc ? p + 1 : 0, hi = asc ? n : p;
                 if (!skip)
-                    for (int i = lo + tid; i < hi; i += kTrsvThreads) x[i] = fma(-xp, col[i], x[i]);
+                    for (int i = lo + tid; i < hi; i += NT) x[i] = fma(-xp, col[i], x[i]);
                 __syncthreads();
                 if (tid == 0) x[p] = xp;
             }
         }
         __syncthreads();
-        for (int i = tid; i < n; i += kTrsvThreads) xg[(size_t)i * incx] = x[i];
+        for (int i = tid; i < n; i += NT) xg[(size_t)i * incx] = x[i];
         __syncthreads();
     }
 }
 
 // One lane per vector: Left -> column v of B (length m), Right -> row v
 // (length n).  The loops are dtrsm.f's, restricted to that vector.
-template <char SIDE, char UPLO, char TRANS>
+// G lanes per matrix (a power of two; 64 / G matrices share the wavefront when
+// a matrix has fewer than 64 vectors).  STAGED: each lane's vector lives in
+// LDS as vec[k][lane] for the whole solve (read and written back once).
+template <char SIDE, char UPLO, char TRANS, bool STAGED>
 __global__ void __launch_bounds__(64) trsm_kernel(int m, int n, double alpha, double **A_array, int lda,
-                                                  double **B_array, int ldb, bool unit, int count)
+                                                  double **B_array, int ldb, bool unit, int count, int G)
 {
-    const int nv = SIDE == 'L' ? n : m;
-    const int vblocks = (nv + 63) / 64;
-    for (int item = blockIdx.x; item < count * vblocks; item += gridDim.x) {
-        const int id = item / vblocks;
-        const int v = (item % vblocks) * 64 + (int)threadIdx.x;
-        if (v >= nv) continue;
+    extern __shared__ double vec[];
+    const int lane = threadIdx.x;
+    const int L = SIDE == 'L' ? m : n;          // vector length
+    const int nv = SIDE == 'L' ? n : m;         // vectors per matrix
+    const int P = 64 / G;
+    const int vblocks = (nv + G - 1) / G;
+    const int groups = (count + P - 1) / P;
+    for (int item = blockIdx.x; item < groups * vblocks; item += gridDim.x) {
+        const int id = (item / vblocks) * P + lane / G;
+        const int v = (item % vblocks) * G + lane % G;
+        if (id >= count || v >= nv) continue;
         const double *a = A_array[id];
         double *b = B_array[id];
 #define AA(i, j) a[(size_t)(i) + (size_t)(j) * lda]
-#define XV(k) (SIDE == 'L' ? b[(size_t)(k) + (size_t)v * ldb] : b[(size_t)v + (size_t)(k) * ldb])
+#define GV(k) (SIDE == 'L' ? b[(size_t)(k) + (size_t)v * ldb] : b[(size_t)v + (size_t)(k) * ldb])
+#define XV(k) (STAGED ? vec[(size_t)(k) * 64 + lane] : GV(k))
         if (alpha == 0.0) {
-            for (int k = 0; k < (SIDE == 'L' ? m : n); ++k) XV(k) = 0.0;
+            for (int k = 0; k < L; ++k) GV(k) = 0.0;
             continue;
         }
+        if (STAGED)
+            for (int k = 0; k < L; ++k) vec[(size_t)k * 64 + lane] = GV(k);
         if (SIDE == 'L') {
             if (TRANS == 'N') {
                 if (alpha != 1.0)
@@ -205,7 +220,10 @@ __global__ void __launch_bounds__(64) trsm_kernel(int m, int n, double alpha, do
                 XV(k) = bk;
             }
         }
+        if (STAGED)
+            for (int k = 0; k < L; ++k) GV(k) = vec[(size_t)k * 64 + lane];
 #undef AA
+#undef GV
 #undef XV
     }
 }
@@ -283,7 +301,13 @@ magma_int_t launch_trsv(int n, double **A, int lda, double **x, int incx, bool u
     int pb = (int)(kTrsvLds / (sizeof(double) * (size_t)n)) - 1;
     pb = std::max(1, std::min(32, pb));
     const size_t lds = sizeof(double) * (size_t)n * (pb + 1);
-    trsv_kernel<T, U><<<persistent_grid(count, 2), kTrsvThreads, lds, st>>>(n, A, lda, x, incx, unit, pb, count);
+    // workgroups resident per CU as the 160 KB of LDS allows
+    const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(16, (160 * 1024) / lds));
+    if (n <= 128)
+        trsv_kernel<T, U, 64><<<persistent_grid(count, per_cu), 64, lds, st>>>(n, A, lda, x, incx, unit, pb, count);
+    else
+        trsv_kernel<T, U, kTrsvThreads><<<persistent_grid(count, per_cu), kTrsvThreads, lds, st>>>(
+            n, A, lda, x, incx, unit, pb, count);
     MAGMA_LAUNCH_CHECK();
     return MAGMA_SUCCESS;
 }
@@ -292,8 +316,18 @@ template <char S, char U, char T>
 magma_int_t launch_trsm(int m, int n, double alpha, double **A, int lda, double **B, int ldb, bool unit, int count,
                         hipStream_t st)
 {
-    const long items = (long)count * (((S == 'L' ? n : m) + 63) / 64);
-    trsm_kernel<S, U, T><<<persistent_grid(items, 8), 64, 0, st>>>(m, n, alpha, A, lda, B, ldb, unit, count);
+    const int L = S == 'L' ? m : n, nv = S == 'L' ? n : m;
+    int G = 1;
+    while (G < 64 && G < nv) G *= 2;
+    const long items = (long)((count + 64 / G - 1) / (64 / G)) * ((nv + G - 1) / G);
+    if (L <= kTrsmStagedMax) {
+        const size_t lds = sizeof(double) * 64 * (size_t)L;
+        trsm_kernel<S, U, T, true><<<persistent_grid(items, 8), 64, lds, st>>>(m, n, alpha, A, lda, B, ldb, unit,
+                                                                                count, G);
+    } else {
+        trsm_kernel<S, U, T, false><<<persistent_grid(items, 8), 64, 0, st>>>(m, n, alpha, A, lda, B, ldb, unit,
+                                                                               count, G);
+    }
     MAGMA_LAUNCH_CHECK();
     return MAGMA_SUCCESS;
 }

@@ -54,6 +54,26 @@ magma_int_t magma_dposv_batched(magma_uplo_t uplo, magma_int_t n, magma_int_t nr
                                 double **dA_array, magma_int_t ldda, double **dB_array, magma_int_t lddb,
                                 magma_int_t *dinfo_array, magma_int_t batchCount, magma_queue_t queue);
 
+/* include/magma_zbatched.h:585-606 -- LU without pivoting (m, n <= 2048) and
+ * its solves; netlib dgetf2's operation order with the pivot fixed on the
+ * diagonal.  info[i] = k > 0: the k-th pivot is exactly zero (the column is
+ * left unscaled and the factorization continues). */
+magma_int_t magma_dgetrf_nopiv_batched(magma_int_t m, magma_int_t n, double **dA_array, magma_int_t ldda,
+                                       magma_int_t *info_array, magma_int_t batchCount, magma_queue_t queue);
+magma_int_t magma_dgetrs_nopiv_batched(magma_trans_t trans, magma_int_t n, magma_int_t nrhs,
+                                       double **dA_array, magma_int_t ldda, double **dB_array, magma_int_t lddb,
+                                       magma_int_t *info_array, magma_int_t batchCount, magma_queue_t queue);
+magma_int_t magma_dgesv_nopiv_batched(magma_int_t n, magma_int_t nrhs, double **dA_array, magma_int_t ldda,
+                                      double **dB_array, magma_int_t lddb, magma_int_t *info_array,
+                                      magma_int_t batchCount, magma_queue_t queue);
+
+/* include/magma_zbatched.h:473-479 -- inverse from the getrf factors (A and
+ * ipiv as magma_dgetrf_batched leaves them) into invA; n <= 2048.  info[i] = k
+ * > 0 when U(k,k) is exactly zero (the inverse is still formed). */
+magma_int_t magma_dgetri_outofplace_batched(magma_int_t n, double **dA_array, magma_int_t ldda,
+                                            magma_int_t **dipiv_array, double **dinvA_array, magma_int_t lddia,
+                                            magma_int_t *info_array, magma_int_t batchCount, magma_queue_t queue);
+
 /* include/magma_zbatched.h:202-208 -- B := alpha inv(op(A)) B (Left) or
  * alpha B inv(op(A)) (Right), A triangular; the operation order of the
  * reference BLAS dtrsm, so results are those of LAPACK's CPU path bit for bit. */

@@ -32,10 +32,14 @@
 //         staged through LDS 32 rows at a time, each thread summing its
 //         column in row order.
 #include <algorithm>
+#include <cfloat>
 
 #include "kernels.h"
 
 namespace magma_amd {
+
+magma_int_t zero_info(int *info, int count, hipStream_t st);
+
 namespace {
 
 constexpr int kTrsvThreads = 256;
@@ -289,6 +293,59 @@ __global__ void __launch_bounds__(64) gemv_t_kernel(int m, int n, double alpha, 
     }
 }
 
+// LU without pivoting, one workgroup per matrix: for each column the
+// multipliers (reciprocal when |pivot| >= DBL_MIN, else division; a zero
+// pivot leaves the column unscaled and sets info), then the rank-1 update
+// column by column with dger's skip of zero pivot-row entries, so every
+// entry receives its updates as single fmas in increasing k.  LDSM: the
+// matrix is held in LDS (m n doubles); otherwise it is updated in place.
+template <bool LDSM>
+__global__ void __launch_bounds__(256) getrf_nopiv_kernel(int m, int n, double **A_array, int lda, int *info,
+                                                         int count)
+{
+    extern __shared__ double sa[];
+    const int tid = threadIdx.x;
+    const int mn = min(m, n);
+    for (int id = blockIdx.x; id < count; id += gridDim.x) {
+        double *ga = A_array[id];
+        double *a = LDSM ? sa : ga;
+        const int ld = LDSM ? m : lda;
+        if (LDSM) {
+            for (int e = tid; e < m * n; e += 256) sa[e] = ga[(size_t)(e % m) + (size_t)(e / m) * lda];
+            __syncthreads();
+        }
+        int inf = 0;
+#pragma unroll 1
+        for (int j = 0; j < mn; ++j) {
+            const double p = a[j + (size_t)j * ld];
+            if (p != 0.0) {
+                const bool rcp = fabs(p) >= DBL_MIN;
+                const double r = 1.0 / p;
+                for (int i = j + 1 + tid; i < m; i += 256) {
+                    double *x = &a[i + (size_t)j * ld];
+                    *x = rcp ? r * *x : *x / p;
+                }
+            } else if (inf == 0) {
+                inf = j + 1;
+            }
+            __syncthreads();
+            if (j < mn - 1) {
+                const int rows = m - j - 1, cols = n - j - 1;
+                for (int e = tid; e < rows * cols; e += 256) {
+                    const int i = j + 1 + e % rows, c = j + 1 + e / rows;
+                    const double y = a[j + (size_t)c * ld];
+                    if (y != 0.0) a[i + (size_t)c * ld] = fma(a[i + (size_t)j * ld], -y, a[i + (size_t)c * ld]);
+                }
+            }
+            __syncthreads();
+        }
+        if (LDSM)
+            for (int e = tid; e < m * n; e += 256) ga[(size_t)(e % m) + (size_t)(e / m) * lda] = sa[e];
+        if (tid == 0 && info) info[id] = inf;
+        __syncthreads();
+    }
+}
+
 magma_int_t sync_unless_async(magma_queue_t queue)
 {
     if (!(queue && queue->async)) MAGMA_HIP_CHECK(hipStreamSynchronize(stream_of(queue)));
@@ -445,6 +502,102 @@ void magmablas_dgemv_batched(magma_trans_t trans, magma_int_t m, magma_int_t n, 
         return;
     }
     (void)sync_unless_async(queue);
+}
+
+// ---------------------------------------------------------- no pivoting ----
+// include/magma_zbatched.h:585-589; argument checks of
+// src/zgetrf_nopiv_batched.cpp:88-99 (m -1, n -2, ldda -4), sizes up to 2048.
+magma_int_t magma_dgetrf_nopiv_batched(magma_int_t m, magma_int_t n, double **dA_array, magma_int_t ldda,
+                                       magma_int_t *info_array, magma_int_t batchCount, magma_queue_t queue)
+{
+    magma_int_t arg = 0;
+    if (m < 0) arg = -1;
+    else if (n < 0) arg = -2;
+    else if (ldda < std::max(1, m)) arg = -4;
+    else if (batchCount < 0) arg = -6;
+    if (arg != 0) {
+        magma_xerbla(__func__, -arg);
+        return arg;
+    }
+    hipStream_t st = stream_of(queue);
+    if (batchCount == 0) return MAGMA_SUCCESS;
+    if (m == 0 || n == 0) {
+        zero_info(info_array, batchCount, st);
+        return sync_unless_async(queue);
+    }
+    if (m > 2048 || n > 2048) return MAGMA_ERR_NOT_SUPPORTED;
+    const size_t lds = sizeof(double) * (size_t)m * n;
+    if (lds <= 64 * 1024) {
+        const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, (160 * 1024) / lds));
+        getrf_nopiv_kernel<true><<<persistent_grid(batchCount, per_cu), 256, lds, st>>>(m, n, dA_array, ldda,
+                                                                                       info_array, batchCount);
+    } else {
+        getrf_nopiv_kernel<false><<<persistent_grid(batchCount, 2), 256, 0, st>>>(m, n, dA_array, ldda, info_array,
+                                                                                 batchCount);
+    }
+    MAGMA_LAUNCH_CHECK();
+    return sync_unless_async(queue);
+}
+
+// include/magma_zbatched.h:593-598: the two triangular solves of getrs
+// without the row interchanges (reference dtrsm order, as dgetrs does).
+magma_int_t magma_dgetrs_nopiv_batched(magma_trans_t trans, magma_int_t n, magma_int_t nrhs, double **dA_array,
+                                       magma_int_t ldda, double **dB_array, magma_int_t lddb,
+                                       magma_int_t *info_array, magma_int_t batchCount, magma_queue_t queue)
+{
+    magma_int_t arg = 0;
+    if (trans != MagmaNoTrans && trans != MagmaTrans && trans != MagmaConjTrans) arg = -1;
+    else if (n < 0) arg = -2;
+    else if (nrhs < 0) arg = -3;
+    else if (ldda < std::max(1, n)) arg = -5;
+    else if (lddb < std::max(1, n)) arg = -7;
+    else if (batchCount < 0) arg = -9;
+    if (arg != 0) {
+        magma_xerbla(__func__, -arg);
+        return arg;
+    }
+    hipStream_t st = stream_of(queue);
+    if (batchCount == 0) return MAGMA_SUCCESS;
+    zero_info(info_array, batchCount, st);
+    if (n == 0 || nrhs == 0) return sync_unless_async(queue);
+    magma_int_t r;
+    if (trans == MagmaNoTrans) {
+        r = launch_trsm<'L', 'L', 'N'>(n, nrhs, 1.0, dA_array, ldda, dB_array, lddb, true, batchCount, st);
+        if (r == MAGMA_SUCCESS)
+            r = launch_trsm<'L', 'U', 'N'>(n, nrhs, 1.0, dA_array, ldda, dB_array, lddb, false, batchCount, st);
+    } else {
+        r = launch_trsm<'L', 'U', 'T'>(n, nrhs, 1.0, dA_array, ldda, dB_array, lddb, false, batchCount, st);
+        if (r == MAGMA_SUCCESS)
+            r = launch_trsm<'L', 'L', 'T'>(n, nrhs, 1.0, dA_array, ldda, dB_array, lddb, true, batchCount, st);
+    }
+    if (r != MAGMA_SUCCESS) return r;
+    return sync_unless_async(queue);
+}
+
+// include/magma_zbatched.h:601-606
+magma_int_t magma_dgesv_nopiv_batched(magma_int_t n, magma_int_t nrhs, double **dA_array, magma_int_t ldda,
+                                      double **dB_array, magma_int_t lddb, magma_int_t *info_array,
+                                      magma_int_t batchCount, magma_queue_t queue)
+{
+    magma_int_t arg = 0;
+    if (n < 0) arg = -1;
+    else if (nrhs < 0) arg = -2;
+    else if (ldda < std::max(1, n)) arg = -4;
+    else if (lddb < std::max(1, n)) arg = -6;
+    else if (batchCount < 0) arg = -8;
+    if (arg != 0) {
+        magma_xerbla(__func__, -arg);
+        return arg;
+    }
+    const bool was_async = queue && queue->async;
+    if (queue) queue->async = true;
+    magma_int_t r = magma_dgetrf_nopiv_batched(n, n, dA_array, ldda, info_array, batchCount, queue);
+    if (r == MAGMA_SUCCESS)
+        r = magma_dgetrs_nopiv_batched(MagmaNoTrans, n, nrhs, dA_array, ldda, dB_array, lddb, nullptr, batchCount,
+                                       queue);
+    if (queue) queue->async = was_async;
+    if (r != MAGMA_SUCCESS) return r;
+    return sync_unless_async(queue);
 }
 
 }  // extern "C"

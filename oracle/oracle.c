@@ -107,6 +107,40 @@ void orc_dgetrf(int m, int n, double *A, int lda, int *ipiv, int *info)
     }
 }
 
+/* LU without pivoting: orc_dgetrf with the pivot search removed (pivot =
+ * the diagonal entry), the same multiplier rule, the same dger zero skip and
+ * the same info rule (first zero pivot; the factorization continues with the
+ * column unscaled).  Semantics of magma_dgetrf_nopiv_batched
+ * (src/zgetrf_nopiv_batched.cpp:76-306); netlib LAPACK has no such routine. */
+void orc_dgetrf_nopiv(int m, int n, double *A, int lda, int *info)
+{
+    *info = 0;
+    const int mn = imin(m, n);
+    const double sfmin = DBL_MIN;
+    for (int j = 0; j < mn; ++j) {
+        const double p = A_(j, j);
+        if (p != 0.0) {
+            if (fabs(p) >= sfmin) {
+                const double r = 1.0 / p;
+                for (int i = j + 1; i < m; ++i) A_(i, j) = r * A_(i, j);
+            } else {
+                for (int i = j + 1; i < m; ++i) A_(i, j) = A_(i, j) / p;
+            }
+        } else if (*info == 0) {
+            *info = j + 1;
+        }
+        if (j < mn - 1) {
+            for (int c = j + 1; c < n; ++c) {
+                const double y = A_(j, c);
+                if (y != 0.0) {
+                    const double t = -y;
+                    for (int i = j + 1; i < m; ++i) A_(i, c) = fma(A_(i, j), t, A_(i, c));
+                }
+            }
+        }
+    }
+}
+
 /* netlib dlaswp: row interchanges k1..k2 (1-based, inclusive) applied in order
  * (incx > 0) or in reverse order (incx < 0).  Used by dgetrs and by the tester's
  * LU check (testing/testing_zgetrf_batched.cpp:47). */

@@ -30,6 +30,7 @@ _PROTO = {
     "orc_dpotrf": (None, [_C, _I, _P, _I, _P]),
     "orc_dpotrs": (None, [_C, _I, _I, _P, _I, _P, _I]),
     "orc_dgemm": (None, [_C, _C, _I, _I, _I, _D, _P, _I, _P, _I, _D, _P, _I]),
+    "orc_dgetrf_nopiv": (None, [_I, _I, _P, _I, _P]),
     "orc_dtrsm": (None, [_C, _C, _C, _C, _I, _I, _D, _P, _I, _P, _I]),
     "orc_dtrsv": (None, [_C, _C, _C, _I, _P, _I, _P, _I]),
     "orc_dgemv": (None, [_C, _I, _I, _D, _P, _I, _P, _I, _D, _P, _I]),
@@ -123,6 +124,17 @@ def dgemm(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc):
     A = _f64(np.array(A, copy=True)); B = _f64(np.array(B, copy=True)); C = _f64(np.array(C, copy=True))
     lib().orc_dgemm(ta, tb, m, n, k, alpha, _p(A), lda, _p(B), ldb, beta, _p(C), ldc)
     return C
+
+
+def dgetrf_nopiv(A, m=None, n=None):
+    """LU without pivoting of one (cols, ld) matrix; returns (LU, info)."""
+    A = _f64(np.array(A, dtype=np.float64, copy=True))
+    cols, ld = A.shape
+    m = ld if m is None else m
+    n = cols if n is None else n
+    info = np.zeros(1, dtype=np.int32)
+    lib().orc_dgetrf_nopiv(m, n, _p(A), ld, _p(info))
+    return A, int(info[0])
 
 
 def dtrsm(side, uplo, trans, diag, m, n, alpha, A, B):

@@ -141,3 +141,127 @@ def test_blas_batched_argument_errors_leave_data(queue):
     L.magmablas_dgemv_batched(ma.MagmaNoTrans, 4, 4, 2.0, dA.ptr, 4, dB.ptr, 0, 0.0, dB.ptr, 1, 2, q)
     queue.sync()
     assert np.array_equal(dB.numpy(), B)
+
+
+@pytest.mark.parametrize("n", [1, 5, 32, 64, 100, 256, 300])
+def test_getri_outofplace_batched(queue, oracle, n):
+    """Inverse from the getrf factors: bitwise the oracle's dgetrs applied to
+    the identity (column j solves L U x = e_p(j), as the reference's column
+    interchange after its two triangular solves produces), with the tester's
+    residual ||I - A inv(A)|| / (n ||A|| ||inv(A)|| eps) small."""
+    import magma_amd as ma
+    import torch
+    L = ma.load()
+    rng = np.random.default_rng(n)
+    count = 6
+    A = rng.uniform(-1, 1, (count, n, n))
+    if n > 1:
+        A[count - 1][2 % n, :] = 0.0                   # a zero column: U(3,3) = 0 -> info 3
+    dA = ma.DeviceBatch(A, ld=n + 1)
+    ret, ipiv, info = ma.dgetrf_batched(dA, queue)
+    assert ret == 0
+    dI = ma.DeviceBatch(np.full((count, n, n), 5.0), ld=n + 2)
+    info2 = torch.full((count,), -9, dtype=torch.int32, device="cuda")
+    print(f"getri n={n}", flush=True)
+    assert L.magma_dgetri_outofplace_batched(n, dA.ptr, dA.ld, ipiv.ptr, dI.ptr, dI.ld, info2.data_ptr(), count,
+                                             queue.handle) == 0
+    LU = dA.numpy(); piv = ipiv.numpy(); inv = dI.numpy(); inf = info2.cpu().numpy(); inf_f = info.cpu().numpy()
+    for s in range(count):
+        want = oracle.dgetrs(LU[s], piv[s], np.eye(n))
+        assert same(inv[s], want), s
+        assert inf[s] == inf_f[s], (s, inf[s], inf_f[s])
+        if inf[s] == 0:
+            Am = A[s].T
+            R = np.eye(n) - Am @ inv[s].T
+            err = np.abs(R).max() / (n * np.abs(Am).max() * np.abs(inv[s]).max() * 2.0 ** -53)
+            assert err < 30, (s, err)
+    if n > 1:
+        assert inf[count - 1] == 3
+
+
+def test_getri_argument_errors(queue):
+    import magma_amd as ma
+    L = ma.load()
+    q = queue.handle
+    assert L.magma_dgetri_outofplace_batched(-1, 0, 1, 0, 0, 1, 0, 1, q) == -1
+    assert L.magma_dgetri_outofplace_batched(4, 0, 3, 0, 0, 4, 0, 1, q) == -3
+    assert L.magma_dgetri_outofplace_batched(4, 0, 4, 0, 0, 3, 0, 1, q) == -6
+
+
+# ------------------------------------------------------------ no pivoting ----
+def _dominant(rng, count, m, n):
+    A = rng.uniform(-1, 1, (count, n, m))
+    k = min(m, n)
+    A[:, np.arange(k), np.arange(k)] += np.sign(A[:, np.arange(k), np.arange(k)]) * (m + 1)
+    return A
+
+
+@pytest.mark.parametrize("m,n", [(1, 1), (5, 5), (32, 32), (64, 64), (90, 90), (100, 100), (256, 256),
+                                 (70, 40), (40, 70)])
+def test_getrf_nopiv_batched_bitexact(queue, oracle, m, n):
+    """Bitwise the oracle's no-pivot dgetf2 (LDS-resident up to 64 KB per
+    matrix, in place above), on dominant and on general matrices (the latter
+    with growth, NaN and Inf compared as values)."""
+    import magma_amd as ma
+    import torch
+    L = ma.load()
+    rng = np.random.default_rng(m * 1000 + n)
+    count = 6
+    A = np.concatenate([_dominant(rng, count // 2, m, n), rng.uniform(-1, 1, (count - count // 2, n, m))])
+    if min(m, n) > 2:
+        A[count - 1][1, :] = 0.0
+        A[count - 1][:, 1] = 0.0                      # second pivot exactly zero -> info 2
+    dA = ma.DeviceBatch(A, ld=m + 3)
+    info = torch.full((count,), -5, dtype=torch.int32, device="cuda")
+    print(f"getrf_nopiv m={m} n={n}", flush=True)
+    assert L.magma_dgetrf_nopiv_batched(m, n, dA.ptr, dA.ld, info.data_ptr(), count, queue.handle) == 0
+    got = dA.numpy(); inf = info.cpu().numpy()
+    for s in range(count):
+        LUo, info_o = oracle.dgetrf_nopiv(A[s], m, n)
+        assert inf[s] == info_o, (s, inf[s], info_o)
+        assert same(got[s], LUo), s
+    if min(m, n) > 2:
+        assert inf[count - 1] == 2
+
+
+@pytest.mark.parametrize("trans", "NT")
+@pytest.mark.parametrize("n,nrhs", [(1, 1), (17, 3), (64, 16), (150, 2)])
+def test_getrs_gesv_nopiv_batched_bitexact(queue, oracle, trans, n, nrhs):
+    import magma_amd as ma
+    import torch
+    L = ma.load()
+    rng = np.random.default_rng(n + nrhs)
+    count = 4
+    A = _dominant(rng, count, n, n)
+    B = rng.uniform(-1, 1, (count, nrhs, n))
+    dA = ma.DeviceBatch(A, ld=n + 1)
+    info = torch.zeros(count, dtype=torch.int32, device="cuda")
+    assert L.magma_dgetrf_nopiv_batched(n, n, dA.ptr, dA.ld, info.data_ptr(), count, queue.handle) == 0
+    dB = ma.DeviceBatch(B, ld=n + 2)
+    et = ma.MagmaNoTrans if trans == "N" else ma.MagmaTrans
+    print(f"getrs_nopiv {trans} n={n} nrhs={nrhs}", flush=True)
+    assert L.magma_dgetrs_nopiv_batched(et, n, nrhs, dA.ptr, dA.ld, dB.ptr, dB.ld, info.data_ptr(), count,
+                                        queue.handle) == 0
+    LU = dA.numpy(); X = dB.numpy()
+    ident = np.arange(1, n + 1, dtype=np.int32)
+    for s in range(count):
+        assert same(X[s], oracle.dgetrs(LU[s], ident, B[s], trans.encode())), s
+    if trans == "N":
+        dA2 = ma.DeviceBatch(A, ld=n + 1)
+        dB2 = ma.DeviceBatch(B, ld=n + 2)
+        assert L.magma_dgesv_nopiv_batched(n, nrhs, dA2.ptr, dA2.ld, dB2.ptr, dB2.ld, info.data_ptr(), count,
+                                           queue.handle) == 0
+        assert same(dA2.numpy(), LU) and same(dB2.numpy(), X)
+        for s in range(count):
+            assert oracle.solve_error(A[s], X[s], B[s]) < 30
+
+
+def test_nopiv_argument_errors(queue):
+    import magma_amd as ma
+    L = ma.load()
+    q = queue.handle
+    assert L.magma_dgetrf_nopiv_batched(-1, 4, 0, 4, 0, 1, q) == -1
+    assert L.magma_dgetrf_nopiv_batched(4, -1, 0, 4, 0, 1, q) == -2
+    assert L.magma_dgetrf_nopiv_batched(4, 4, 0, 3, 0, 1, q) == -4
+    assert L.magma_dgetrs_nopiv_batched(ma.MagmaNoTrans, 4, 1, 0, 3, 0, 4, 0, 1, q) == -5
+    assert L.magma_dgesv_nopiv_batched(4, 1, 0, 4, 0, 3, 0, 1, q) == -6

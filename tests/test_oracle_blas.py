@@ -159,3 +159,45 @@ def test_gemv_matches_openblas(oracle, trans):
     # the gemv tester's bound (testing/testing_zgemv_batched.cpp:173-183)
     bound = 3 * EPS * (np.sqrt(k + 2) * np.linalg.norm(A) * np.linalg.norm(x) + 2 * np.linalg.norm(y))
     assert np.linalg.norm(got - ref) <= bound
+
+
+# ------------------------------------------------------------ LU, no pivots --
+@pytest.mark.parametrize("m,n", [(6, 6), (9, 5), (5, 9), (1, 1)])
+def test_getrf_nopiv_integer_exact(oracle, m, n):
+    """A = L U with L unit lower and U upper of small integers, diagonal of U
+    in {1, -1, 2, -2}: the factorization returns L and U exactly."""
+    rng = np.random.default_rng(m * 10 + n)
+    k = min(m, n)
+    Lm = np.tril(rng.integers(-3, 4, (m, k)).astype(np.float64), -1)
+    Lm[np.arange(k), np.arange(k)] = 1.0
+    Um = np.triu(rng.integers(-3, 4, (k, n)).astype(np.float64))
+    Um[np.arange(k), np.arange(k)] = rng.choice([1.0, -1.0, 2.0, -2.0], k)
+    A = Lm @ Um
+    LU, info = oracle.dgetrf_nopiv(np.ascontiguousarray(A.T))
+    got = LU.T
+    assert info == 0
+    assert np.array_equal(np.tril(got[:, :k], -1), np.tril(Lm, -1)[:, :k])
+    assert np.array_equal(np.triu(got[:k, :]), Um)
+
+
+def test_getrf_nopiv_equals_pivoted_when_no_swap(oracle):
+    """Column-diagonally dominant matrices: partial pivoting never swaps, so
+    the no-pivot factorization is the pivoted one bit for bit."""
+    rng = np.random.default_rng(3)
+    n = 40
+    A = rng.uniform(-1, 1, (n, n))
+    A[np.arange(n), np.arange(n)] = np.abs(A).sum(axis=0) + 1.0
+    S = np.ascontiguousarray(A.T)
+    LUp, ipiv, infop = oracle.dgetrf(S)
+    assert np.array_equal(ipiv, np.arange(1, n + 1))
+    LUn, info = oracle.dgetrf_nopiv(S)
+    assert info == infop == 0 and np.array_equal(LUn, LUp)
+
+
+def test_getrf_nopiv_zero_pivot_info(oracle):
+    A = np.array([[0.0, 1.0], [1.0, 1.0]])          # A(0,0) = 0
+    LU, info = oracle.dgetrf_nopiv(A.T.copy())
+    assert info == 1
+    B = np.array([[2.0, 1.0, 0.0], [4.0, 2.0, 1.0], [1.0, 3.0, 5.0]])   # second pivot 2 - 2 = 0
+    LU, info = oracle.dgetrf_nopiv(B.T.copy())
+    assert info == 2

@@ -1,7 +1,8 @@
 // gemm.hip -- batched and vbatched DGEMM, C = alpha op(A) op(B) + beta C, on
 // FP64 MFMA (v_mfma_f64_16x16x4f64).
 //
-// Each wavefront owns one 32x32 tile of C (2x2 MFMA blocks of 16x16) and
+// Each wavefront owns one 16x16, 32x32 or 48x48 tile of C (1, 4 or 9 MFMA
+// blocks of 16x16; the variant is tuned per size class) and
 // walks k four at a time; a workgroup holds 4 independent wavefronts.  The
 // grid is persistent: wavefronts stride over a flat tile list.  For vbatched
 // calls the list is the device prefix sum of per-matrix tile counts
@@ -27,7 +28,7 @@ constexpr int kWaves = 4;
 __device__ __forceinline__ int pick(const int *arr, int fixed, int id) { return arr ? arr[id] : fixed; }
 
 template <bool TA, bool TB, int BT>
-__global__ void __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(BT == 1 ? 4 : 3))) gemm_mfma_kernel(GemmSet g, const int *prefix, int fixed_tiles)
+__global__ void __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(BT == 1 ? 4 : (BT == 2 ? 3 : 2)))) gemm_mfma_kernel(GemmSet g, const int *prefix, int fixed_tiles)
 {
     constexpr int TS = 16 * BT;        // tile edge: BT x BT MFMA blocks of 16x16
     const int lane = threadIdx.x & 63;
@@ -74,13 +75,17 @@ __global__ void __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_e
         // A^T) are one contiguous 32-byte piece.  Rows and columns past the edge
         // load a valid address and multiply by zero.
         const int kmain = k & ~15;
-        const int rq0 = min(row0 + li, m - 1), rq1 = min(row0 + 16 * (BT - 1) + li, m - 1);
-        const int cq0 = min(col0 + li, n - 1), cq1 = min(col0 + 16 * (BT - 1) + li, n - 1);
+        int rq[BT], cq[BT];
+#pragma unroll
+        for (int q = 0; q < BT; ++q) {
+            rq[q] = min(row0 + 16 * q + li, m - 1);
+            cq[q] = min(col0 + 16 * q + li, n - 1);
+        }
         double av[2][BT][4], bv[2][BT][4];
         auto load16 = [&](int kb, double (&ao)[BT][4], double (&bo)[BT][4]) {
 #pragma unroll
             for (int q = 0; q < BT; ++q) {
-                const int r = q == 0 ? rq0 : rq1, c = q == 0 ? cq0 : cq1;
+                const int r = rq[q], c = cq[q];
                 const bool rin = row0 + 16 * q + li < m, cin = col0 + 16 * q + li < n;
 #pragma unroll
                 for (int sl = 0; sl < 4; ++sl) {
@@ -200,11 +205,11 @@ static void launch_bt(const GemmSet &g, const int *prefix, int fixed_tiles, int 
 }
 
 // prefix: device tile prefix (vbatched) or null (fixed sizes), built for tile
-// edge `tile` (16 or 32).  max_tiles bounds the flat tile count for the grid size.
+// edge `tile` (16, 32 or 48).  max_tiles bounds the flat tile count for the grid size.
 magma_int_t launch_gemm_tiles(const GemmSet &g, const int *prefix, long max_tiles, int tile, hipStream_t st)
 {
     if (g.count <= 0 || max_tiles <= 0) return MAGMA_SUCCESS;
-    if (tile != 16 && tile != 32) return MAGMA_ERR_NOT_SUPPORTED;
+    if (tile != 16 && tile != 32 && tile != 48) return MAGMA_ERR_NOT_SUPPORTED;
     const int fixed_tiles = prefix ? 0 : ceil_div(g.m, tile) * ceil_div(g.n, tile);
     // enough resident wavefronts to fill 256 CUs several times over
     const long want = (max_tiles + kWaves - 1) / kWaves;
@@ -212,6 +217,8 @@ magma_int_t launch_gemm_tiles(const GemmSet &g, const int *prefix, long max_tile
     const bool scale_only = (!prefix && (g.k == 0 || g.alpha == 0.0)) || g.alpha == 0.0;
     if (scale_only) {
         gemm_scale_kernel<<<grid, 256, 0, st>>>(g, prefix, fixed_tiles, tile);
+    } else if (tile == 48) {
+        launch_bt<3>(g, prefix, fixed_tiles, grid, st);
     } else if (tile == 32) {
         launch_bt<2>(g, prefix, fixed_tiles, grid, st);
     } else {

@@ -31,7 +31,7 @@ void init_table()
     g_init = true;
 }
 
-const int kVariants[kTuneFamilies] = {3, 5, 2};
+const int kVariants[kTuneFamilies] = {3, 5, 3};
 
 __global__ void fill_kernel(double *p, size_t count, unsigned seed)
 {
@@ -252,7 +252,7 @@ magma_int_t tune_gemm(magma_queue_t queue, FILE *log)
     float best = 1e30f;
     int bestv = 0;
     for (int v = 0; v < kVariants[kTuneGemm]; ++v) {
-        const int tile = v == 0 ? 32 : 16;
+        const int tile = gemm_tile_of(v);
         if (build_tile_prefix(g.m_arr, g.n_arr, count, tile, tile, prefix.get<int>(), st) != MAGMA_SUCCESS)
             return MAGMA_ERR_UNKNOWN;
         const long tiles = (long)ceil_div(256, tile) * ceil_div(256, tile) * count;
@@ -277,7 +277,7 @@ magma_int_t tune_gemm(magma_queue_t queue, FILE *log)
         best = 1e30f;
         bestv = 0;
         for (int v = 0; v < kVariants[kTuneGemm]; ++v) {
-            const int tile = v == 0 ? 32 : 16;
+            const int tile = gemm_tile_of(v);
             const long tiles = (long)ceil_div(n, tile) * ceil_div(n, tile) * f.count;
             const float ms = best_ms(st, 3, [&]() {}, [&]() { (void)launch_gemm_tiles(f, nullptr, tiles, tile, st); });
             if (log) std::fprintf(log, "autotune gemm n=%d x %d tile %d: %.3f ms\n", n, f.count, tile, ms);

@@ -602,7 +602,7 @@ void magmablas_dgemm_batched(magma_trans_t transA, magma_trans_t transB,
     g.alpha = alpha; g.beta = beta;
     g.A = dA_array; g.B = dB_array; g.C = dC_array;
     g.count = batchCount;
-    const int tile = tuned_variant(kTuneGemm, size_class_of(std::max(m, n))) == 1 ? 16 : 32;
+    const int tile = gemm_tile_of(tuned_variant(kTuneGemm, size_class_of(std::max(m, n))));
     const long tiles = (long)ceil_div(m, tile) * ceil_div(n, tile) * batchCount;
     launch_gemm_tiles(g, nullptr, tiles, tile, stream_of(queue));
 }
@@ -637,7 +637,7 @@ void magmablas_dgemm_vbatched_max_nocheck(magma_trans_t transA, magma_trans_t tr
     // one tile size for the whole call: the class of the largest matrix, or the
     // mixed-size class when the caller's bounds are not tight
     const int cls = (max_m > 2048 || max_n > 2048) ? kTuneClasses - 1 : size_class_of(std::max(max_m, max_n));
-    const int tile = tuned_variant(kTuneGemm, cls) == 1 ? 16 : 32;
+    const int tile = gemm_tile_of(tuned_variant(kTuneGemm, cls));
     if (build_tile_prefix(m, n, batchCount, tile, tile, v.prefix, st) != MAGMA_SUCCESS) return;
     GemmSet g;
     g.ta = tr(transA); g.tb = tr(transB);

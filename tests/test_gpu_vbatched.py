@@ -224,3 +224,50 @@ def test_vbatched_argument_errors(queue):
                                    queue.handle) == -2
     assert L.magma_dpotrf_vbatched(ma.MagmaUpper, n2.data_ptr(), 0, ld.data_ptr(), info.data_ptr(), 3,
                                    queue.handle) == -1
+
+
+def test_gemm_tile_variants_bitwise_equal(queue, oracle):
+    """Every C entry is accumulated over k in the same order whatever the tile
+    edge (16, 32, 48), so the three GEMM variants agree bit for bit; each also
+    meets the tester's bound and is exact on integer operands."""
+    import magma_amd as ma
+    from magma_amd import gen
+    L = ma.load()
+    sizes = [int(n) for n in sizes_cfg3(120)] + [1, 7, 33, 47, 48, 49, 97, 129, 256]
+    As = [gen.uniform_pm1(3, [i], n * n).reshape(n, n) for i, n in enumerate(sizes)]
+    Bs = [gen.uniform_pm1(103, [i], n * n).reshape(n, n) for i, n in enumerate(sizes)]
+    Cs = [gen.uniform_pm1(203, [i], n * n).reshape(n, n) for i, n in enumerate(sizes)]
+    rng = np.random.default_rng(9)
+    Ai = rng.integers(-4, 5, (30, 50, 61)).astype(float)     # m = 61, k = 50
+    Bi = rng.integers(-4, 5, (30, 53, 50)).astype(float)     # n = 53
+    Ci = rng.integers(-4, 5, (30, 53, 61)).astype(float)
+    outs = []
+    try:
+        for v in range(3):
+            for c in range(10):
+                assert L.magma_amd_tune_set(2, c, v) == 0
+            dA, dB, dC = ma.VarBatch(As), ma.VarBatch(Bs), ma.VarBatch(Cs)
+            print(f"gemm variant {v}", flush=True)
+            L.magmablas_dgemm_vbatched(ma.MagmaNoTrans, ma.MagmaNoTrans, dC.d_rows.data_ptr(), dC.d_cols.data_ptr(),
+                                       dA.d_cols.data_ptr(), 1.0, dA.ptr, dA.d_ld.data_ptr(), dB.ptr,
+                                       dB.d_ld.data_ptr(), 1.0, dC.ptr, dC.d_ld.data_ptr(), dA.count, queue.handle)
+            queue.sync()
+            got = dC.matrices()
+            outs.append(got)
+            worst = max(_gemm_ref_error(As[i], Bs[i], Cs[i], got[i], 1.0, 1.0, oracle, n, n, n)
+                        for i, n in enumerate(sizes))
+            assert worst < 3 * EPS, (v, worst)
+            dAi, dBi, dCi = ma.DeviceBatch(Ai), ma.DeviceBatch(Bi), ma.DeviceBatch(Ci)
+            L.magma_dgemm_batched(ma.MagmaNoTrans, ma.MagmaNoTrans, 61, 53, 50, 2.0, dAi.ptr, dAi.ld, dBi.ptr,
+                                  dBi.ld, -1.0, dCi.ptr, dCi.ld, 30, queue.handle)
+            queue.sync()
+            gi = dCi.numpy()
+            for s in range(30):
+                ref = oracle.dgemm(b"N", b"N", 61, 53, 50, 2.0, Ai[s], 61, Bi[s], 50, -1.0, Ci[s], 61)
+                assert np.array_equal(gi[s], ref), (v, s)
+    finally:
+        for c in range(10):
+            L.magma_amd_tune_set(2, c, -1)
+    for v in (1, 2):
+        for i in range(len(sizes)):
+            assert np.array_equal(outs[v][i], outs[0][i]), (v, sizes[i])

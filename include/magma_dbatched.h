@@ -67,6 +67,17 @@ magma_int_t magma_dgesv_nopiv_batched(magma_int_t n, magma_int_t nrhs, double **
                                       double **dB_array, magma_int_t lddb, magma_int_t *info_array,
                                       magma_int_t batchCount, magma_queue_t queue);
 
+/* include/magma_zbatched.h:609-614 -- solve by a depth-2 random butterfly
+ * transformation (A <- U' A V, B <- U' B), LU without pivoting and X = V Y;
+ * n <= 2048.  The butterfly weights are a fixed function of n.  Unlike the
+ * reference, the caller's ldda is honoured, every right-hand side is
+ * transformed, and the transform is exact for every n (the reference's is
+ * consistent only for n a multiple of 4).  Synchronizes once to stage the
+ * weights. */
+magma_int_t magma_dgesv_rbt_batched(magma_int_t n, magma_int_t nrhs, double **dA_array, magma_int_t ldda,
+                                    double **dB_array, magma_int_t lddb, magma_int_t *dinfo_array,
+                                    magma_int_t batchCount, magma_queue_t queue);
+
 /* include/magma_zbatched.h:473-479 -- inverse from the getrf factors (A and
  * ipiv as magma_dgetrf_batched leaves them) into invA; n <= 2048.  info[i] = k
  * > 0 when U(k,k) is exactly zero (the inverse is still formed). */

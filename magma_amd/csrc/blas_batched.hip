@@ -33,6 +33,8 @@
 //         column in row order.
 #include <algorithm>
 #include <cfloat>
+#include <cmath>
+#include <cstdint>
 
 #include "kernels.h"
 
@@ -346,6 +348,105 @@ __global__ void __launch_bounds__(256) getrf_nopiv_kernel(int m, int n, double *
     }
 }
 
+// Random butterfly transformation, depth 2 (magmablas/zgerbt_func_batched.cu):
+// A <- U' A V with U = U1 U2, V = V1 V2.  Level 1 acts on the two halves
+// (weights n.. of u and v), level 2 on the whole order (weights 0..).  A
+// butterfly of order N pairs index x with x + N/2: (r_x, r_y) <- (w_x (r_x + r_y),
+// w_y (r_x - r_y)); an odd leftover index is unchanged.  Each level is a pass
+// over the rows (every column) and one over the columns (every row), so the
+// transform is exact for every n (the reference's kernels update only the
+// paired square, consistent only when both orders are multiples of 4).  One
+// workgroup per matrix, in place; the passes are separated by barriers.
+__global__ void __launch_bounds__(256) rbt_matrix_kernel(int n, double **A_array, int lda, const double *u,
+                                                        const double *v, int count)
+{
+    const int tid = threadIdx.x;
+    const int h = n / 2, hh = h / 2;
+    for (int id = blockIdx.x; id < count; id += gridDim.x) {
+        double *a = A_array[id];
+#define AE(i, j) a[(size_t)(i) + (size_t)(j) * lda]
+        for (int pass = 0; pass < 4; ++pass) {
+            const bool rows = (pass & 1) == 0, lvl1 = pass < 2;
+            const int pairs = lvl1 ? hh : h;          // pairs per block
+            const int blocks = lvl1 ? 2 : 1;
+            const int items = blocks * pairs * n;
+            for (int e = tid; e < items; e += 256) {
+                // the paired index runs fastest: rows passes read down a column
+                const int x = e % pairs, rest = e / pairs;
+                const int other = rest % n, b = rest / n;
+                const int o = lvl1 ? b * h : 0;
+                const int N2 = lvl1 ? hh : h;         // pair distance
+                const double *w = lvl1 ? ((rows ? u : v) + n + b * h) : (rows ? u : v);
+                const int i0 = o + x, i1 = o + x + N2;
+                double *p0 = rows ? &AE(i0, other) : &AE(other, i0);
+                double *p1 = rows ? &AE(i1, other) : &AE(other, i1);
+                const double s0 = *p0, s1 = *p1;
+                *p0 = w[x] * (s0 + s1);
+                *p1 = w[x + N2] * (s0 - s1);
+            }
+            __syncthreads();
+        }
+#undef AE
+    }
+}
+
+// The right-hand sides: forward (b <- U' b: level 1 on the halves, then level
+// 2), or backward (x <- V y: level 2, then level 1), every column.
+__global__ void __launch_bounds__(256) rbt_vector_kernel(int n, int nrhs, double **B_array, int ldb,
+                                                        const double *w, bool forward, int count)
+{
+    const int tid = threadIdx.x;
+    const int h = n / 2, hh = h / 2;
+    for (int id = blockIdx.x; id < count; id += gridDim.x) {
+        double *bm = B_array[id];
+        for (int step = 0; step < 2; ++step) {
+            const bool lvl1 = forward ? step == 0 : step == 1;
+            const int pairs = lvl1 ? hh : h;
+            const int blocks = lvl1 ? 2 : 1;
+            const int items = blocks * pairs * nrhs;
+            for (int e = tid; e < items; e += 256) {
+                const int x = e % pairs, rest = e / pairs;
+                const int j = rest % nrhs, blk = rest / nrhs;
+                const int o = lvl1 ? blk * h : 0;
+                const int N2 = lvl1 ? hh : h;
+                const double *ww = lvl1 ? w + n + blk * h : w;
+                double *b = bm + (size_t)j * ldb + o;
+                if (forward) {
+                    const double a1 = b[x] + b[x + N2], a2 = b[x] - b[x + N2];
+                    b[x] = ww[x] * a1;
+                    b[x + N2] = ww[x + N2] * a2;
+                } else {
+                    const double a1 = ww[x] * b[x], a2 = ww[x + N2] * b[x + N2];
+                    b[x] = a1 + a2;
+                    b[x + N2] = a1 - a2;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// The butterfly weights: 2n each of u and v, exp((r - 1/2) / 10) with r in
+// [0,1) (the distribution of src/zgerbt_gpu.cpp:16-28) from splitmix64 of
+// 2^32 + 2 i (u) and 2^32 + 2 i + 1 (v): a fixed function of n.
+uint64_t rbt_mix(uint64_t x)
+{
+    uint64_t z = x + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+void rbt_butterflies(int n, double *u, double *v)
+{
+    for (int i = 0; i < 2 * n; ++i) {
+        const double ru = (double)(rbt_mix((1ull << 32) + 2ull * i) >> 11) * 0x1.0p-53;
+        const double rv = (double)(rbt_mix((1ull << 32) + 2ull * i + 1) >> 11) * 0x1.0p-53;
+        u[i] = std::exp((ru - 0.5) / 10.0);
+        v[i] = std::exp((rv - 0.5) / 10.0);
+    }
+}
+
 magma_int_t sync_unless_async(magma_queue_t queue)
 {
     if (!(queue && queue->async)) MAGMA_HIP_CHECK(hipStreamSynchronize(stream_of(queue)));
@@ -597,6 +698,57 @@ magma_int_t magma_dgesv_nopiv_batched(magma_int_t n, magma_int_t nrhs, double **
                                        queue);
     if (queue) queue->async = was_async;
     if (r != MAGMA_SUCCESS) return r;
+    return sync_unless_async(queue);
+}
+
+// include/magma_zbatched.h:609-614; src/zgesv_rbt_batched.cpp: A <- U' A V,
+// B <- U' B (every column), LU without pivoting, the two triangular solves,
+// X = V Y in B.  The butterfly weights are staged through the queue's pinned
+// buffer, so the call synchronizes once before its kernels.
+magma_int_t magma_dgesv_rbt_batched(magma_int_t n, magma_int_t nrhs, double **dA_array, magma_int_t ldda,
+                                    double **dB_array, magma_int_t lddb, magma_int_t *dinfo_array,
+                                    magma_int_t batchCount, magma_queue_t queue)
+{
+    magma_int_t arg = 0;
+    if (n < 0) arg = -1;
+    else if (nrhs < 0) arg = -2;
+    else if (ldda < std::max(1, n)) arg = -4;
+    else if (lddb < std::max(1, n)) arg = -6;
+    else if (batchCount < 0) arg = -8;
+    if (arg != 0) {
+        magma_xerbla(__func__, -arg);
+        return arg;
+    }
+    hipStream_t st = stream_of(queue);
+    if (batchCount == 0) return MAGMA_SUCCESS;
+    if (n == 0 || nrhs == 0) {
+        zero_info(dinfo_array, batchCount, st);
+        return sync_unless_async(queue);
+    }
+    if (n > 2048) return MAGMA_ERR_NOT_SUPPORTED;
+    const size_t bytes = sizeof(double) * 4 * (size_t)n;
+    double *host = static_cast<double *>(queue_host(queue, bytes));
+    double *dev = static_cast<double *>(queue_workspace(queue, bytes));
+    if (!host || !dev) return MAGMA_ERR_DEVICE_ALLOC;
+    MAGMA_HIP_CHECK(hipStreamSynchronize(st));      // the pinned buffer may still feed an earlier copy
+    rbt_butterflies(n, host, host + 2 * n);
+    MAGMA_HIP_CHECK(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, st));
+    const double *du = dev, *dv = dev + 2 * n;
+    const int grid = persistent_grid(batchCount, 4);
+    rbt_matrix_kernel<<<grid, 256, 0, st>>>(n, dA_array, ldda, du, dv, batchCount);
+    MAGMA_LAUNCH_CHECK();
+    rbt_vector_kernel<<<grid, 256, 0, st>>>(n, nrhs, dB_array, lddb, du, true, batchCount);
+    MAGMA_LAUNCH_CHECK();
+    const bool was_async = queue && queue->async;
+    if (queue) queue->async = true;
+    magma_int_t r = magma_dgetrf_nopiv_batched(n, n, dA_array, ldda, dinfo_array, batchCount, queue);
+    if (r == MAGMA_SUCCESS)
+        r = magma_dgetrs_nopiv_batched(MagmaNoTrans, n, nrhs, dA_array, ldda, dB_array, lddb, nullptr, batchCount,
+                                       queue);
+    if (queue) queue->async = was_async;
+    if (r != MAGMA_SUCCESS) return r;
+    rbt_vector_kernel<<<grid, 256, 0, st>>>(n, nrhs, dB_array, lddb, dv, false, batchCount);
+    MAGMA_LAUNCH_CHECK();
     return sync_unless_async(queue);
 }
 

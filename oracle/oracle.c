@@ -311,6 +311,119 @@ void orc_dgemm(char transa, char transb, int m, int n, int k, double alpha,
     }
 }
 
+/* ---- random butterfly transformation (RBT), depth 2 ------------------------
+ * A <- U' A V and b <- U' b with U = U1 U2, V = V1 V2 products of butterfly
+ * matrices, as in magmablas/zgerbt_func_batched.cu: level 1 acts on the two
+ * halves (entries n.. of u and v), level 2 on the whole order (entries 0..).
+ * One butterfly of order N with weights w pairs index x with x + N/2
+ * (x < N/2): (r_x, r_{x+N/2}) <- (w_x (r_x + r_{x+N/2}), w_{x+N/2} (r_x - r_{x+N/2})),
+ * an odd leftover index unchanged.  Each level is one pass over the rows
+ * (every column) and one over the columns (every row).  The reference's
+ * kernels update only the paired square, which is a consistent transform only
+ * when both orders are multiples of 4; this form is exact for every n. */
+static void rbt_rows(int N, int r0, double *A, int lda, int ncols, const double *u)
+{
+    const int h = N / 2;
+    for (int c = 0; c < ncols; ++c) {
+        for (int x = 0; x < h; ++x) {
+            const double a = A_(r0 + x, c), b = A_(r0 + x + h, c);
+            A_(r0 + x, c) = u[x] * (a + b);
+            A_(r0 + x + h, c) = u[x + h] * (a - b);
+        }
+    }
+}
+
+static void rbt_cols(int N, int c0, double *A, int lda, int nrows, const double *v)
+{
+    const int h = N / 2;
+    for (int y = 0; y < h; ++y) {
+        for (int r = 0; r < nrows; ++r) {
+            const double a = A_(r, c0 + y), b = A_(r, c0 + y + h);
+            A_(r, c0 + y) = v[y] * (a + b);
+            A_(r, c0 + y + h) = v[y + h] * (a - b);
+        }
+    }
+}
+
+/* b <- U' b, one butterfly (the row pass on a single column) */
+static void rbt_mtv(int n, const double *u, double *b)
+{
+    const int h = n / 2;
+    for (int x = 0; x < h; ++x) {
+        const double a1 = b[x] + b[x + h], a2 = b[x] - b[x + h];
+        b[x] = u[x] * a1;
+        b[x + h] = u[x + h] * a2;
+    }
+}
+
+/* x <- V y, one butterfly (magmablas/zgerbt_kernels.cu apply_vector form) */
+static void rbt_mv(int n, const double *v, double *b)
+{
+    const int h = n / 2;
+    for (int x = 0; x < h; ++x) {
+        const double a1 = v[x] * b[x], a2 = v[x + h] * b[x + h];
+        b[x] = a1 + a2;
+        b[x + h] = a1 - a2;
+    }
+}
+
+/* The butterfly entries: 2n values each of u and v, exp((r - 1/2) / 10) with
+ * r in [0,1) (the distribution of src/zgerbt_gpu.cpp:16-28), r from
+ * splitmix64 of seed 2^32 + 2 i (u) and 2^32 + 2 i + 1 (v): a fixed
+ * function of n, so every call and every matrix uses the same transform. */
+static uint64_t rbt_mix(uint64_t x)
+{
+    uint64_t z = x + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+void orc_rbt_butterflies(int n, double *u, double *v)
+{
+    for (int i = 0; i < 2 * n; ++i) {
+        const double ru = (double)(rbt_mix((1ull << 32) + 2ull * i) >> 11) * 0x1.0p-53;
+        const double rv = (double)(rbt_mix((1ull << 32) + 2ull * i + 1) >> 11) * 0x1.0p-53;
+        u[i] = exp((ru - 0.5) / 10.0);
+        v[i] = exp((rv - 0.5) / 10.0);
+    }
+}
+
+/* dgesv_rbt: A <- U' A V, B <- U' B (every column), LU without pivoting,
+ * the two triangular solves, X = V Y in B.  A and B as magma_dgesv_rbt_batched
+ * (src/zgesv_rbt_batched.cpp) leaves them. */
+void orc_dgesv_rbt(int n, int nrhs, double *A, int lda, double *B, int ldb, int *info)
+{
+    *info = 0;
+    if (n == 0) return;
+    double *u = (double *)malloc(sizeof(double) * 4 * (size_t)n);
+    double *v = u + 2 * n;
+    orc_rbt_butterflies(n, u, v);
+    const int h = n / 2;
+    rbt_rows(h, 0, A, lda, n, u + n);
+    rbt_rows(h, h, A, lda, n, u + n + h);
+    rbt_cols(h, 0, A, lda, n, v + n);
+    rbt_cols(h, h, A, lda, n, v + n + h);
+    rbt_rows(n, 0, A, lda, n, u);
+    rbt_cols(n, 0, A, lda, n, v);
+    for (int j = 0; j < nrhs; ++j) {
+        double *b = &B_(0, j);
+        rbt_mtv(h, u + n, b);
+        rbt_mtv(h, u + n + h, b + h);
+        rbt_mtv(n, u, b);
+    }
+    orc_dgetrf_nopiv(n, n, A, lda, info);
+    trsm_left('L', 'N', 'U', n, nrhs, A, lda, B, ldb);
+    trsm_left('U', 'N', 'N', n, nrhs, A, lda, B, ldb);
+    for (int j = 0; j < nrhs; ++j) {
+        double *b = &B_(0, j);
+        rbt_mv(n, v, b);
+        rbt_mv(h, v + n, b);
+        rbt_mv(h, v + n + h, b + h);
+    }
+    free(u);
+}
+
 /* reference-BLAS dtrsm (dtrsm.f), all eight side/uplo/transpose cases:
  *   Left:  B := alpha * inv(op(A)) * B,   Right: B := alpha * B * inv(op(A)).
  * Loop orders, the zero skips, the alpha placement, and division (Left) versus

@@ -31,6 +31,8 @@ _PROTO = {
     "orc_dpotrs": (None, [_C, _I, _I, _P, _I, _P, _I]),
     "orc_dgemm": (None, [_C, _C, _I, _I, _I, _D, _P, _I, _P, _I, _D, _P, _I]),
     "orc_dgetrf_nopiv": (None, [_I, _I, _P, _I, _P]),
+    "orc_rbt_butterflies": (None, [_I, _P, _P]),
+    "orc_dgesv_rbt": (None, [_I, _I, _P, _I, _P, _I, _P]),
     "orc_dtrsm": (None, [_C, _C, _C, _C, _I, _I, _D, _P, _I, _P, _I]),
     "orc_dtrsv": (None, [_C, _C, _C, _I, _P, _I, _P, _I]),
     "orc_dgemv": (None, [_C, _I, _I, _D, _P, _I, _P, _I, _D, _P, _I]),
@@ -135,6 +137,22 @@ def dgetrf_nopiv(A, m=None, n=None):
     info = np.zeros(1, dtype=np.int32)
     lib().orc_dgetrf_nopiv(m, n, _p(A), ld, _p(info))
     return A, int(info[0])
+
+
+def rbt_butterflies(n):
+    u = np.zeros(max(2 * n, 1)); v = np.zeros(max(2 * n, 1))
+    lib().orc_rbt_butterflies(n, _p(u), _p(v))
+    return u[: 2 * n], v[: 2 * n]
+
+
+def dgesv_rbt(A, B):
+    """A: (n, lda) storage, B: (nrhs, ldb) storage; returns (A', X, info)."""
+    A = _f64(np.array(A, copy=True)); B = _f64(np.array(B, copy=True))
+    n, lda = A.shape
+    nrhs, ldb = B.shape
+    info = np.zeros(1, dtype=np.int32)
+    lib().orc_dgesv_rbt(n, nrhs, _p(A), lda, _p(B), ldb, _p(info))
+    return A, B, int(info[0])
 
 
 def dtrsm(side, uplo, trans, diag, m, n, alpha, A, B):

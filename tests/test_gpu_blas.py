@@ -265,3 +265,41 @@ def test_nopiv_argument_errors(queue):
     assert L.magma_dgetrf_nopiv_batched(4, 4, 0, 3, 0, 1, q) == -4
     assert L.magma_dgetrs_nopiv_batched(ma.MagmaNoTrans, 4, 1, 0, 3, 0, 4, 0, 1, q) == -5
     assert L.magma_dgesv_nopiv_batched(4, 1, 0, 4, 0, 3, 0, 1, q) == -6
+
+
+@pytest.mark.parametrize("n,nrhs", [(1, 1), (2, 3), (3, 1), (5, 2), (6, 1), (8, 4), (31, 2), (64, 3), (100, 1),
+                                    (255, 2), (256, 1)])
+def test_gesv_rbt_batched_bitexact(queue, oracle, n, nrhs):
+    """Butterfly transform, no-pivot LU and solves: A and X bitwise the
+    oracle's (same weights, same operation order); X meets the RBT tester's
+    bound ||B - A X|| / (n ||A|| ||X|| eps) < 30 (testing/testing_zgesv_rbt.cpp:101-103)."""
+    import magma_amd as ma
+    import torch
+    L = ma.load()
+    rng = np.random.default_rng(n * 7 + nrhs)
+    count = 5
+    A = rng.uniform(-1, 1, (count, n, n))
+    B = rng.uniform(-1, 1, (count, nrhs, n))
+    dA = ma.DeviceBatch(A, ld=n + 3)
+    dB = ma.DeviceBatch(B, ld=n + 1)
+    info = torch.full((count,), -4, dtype=torch.int32, device="cuda")
+    print(f"gesv_rbt n={n} nrhs={nrhs}", flush=True)
+    assert L.magma_dgesv_rbt_batched(n, nrhs, dA.ptr, dA.ld, dB.ptr, dB.ld, info.data_ptr(), count,
+                                     queue.handle) == 0
+    Ag = dA.numpy(); X = dB.numpy(); inf = info.cpu().numpy()
+    for s in range(count):
+        Ao, Xo, info_o = oracle.dgesv_rbt(A[s], B[s])
+        assert inf[s] == info_o == 0
+        assert same(Ag[s], Ao), s
+        assert same(X[s], Xo), s
+        assert oracle.solve_error(A[s], X[s], B[s]) < 30, s
+
+
+def test_gesv_rbt_argument_errors(queue):
+    import magma_amd as ma
+    L = ma.load()
+    q = queue.handle
+    assert L.magma_dgesv_rbt_batched(-1, 1, 0, 1, 0, 1, 0, 1, q) == -1
+    assert L.magma_dgesv_rbt_batched(4, -1, 0, 4, 0, 4, 0, 1, q) == -2
+    assert L.magma_dgesv_rbt_batched(4, 1, 0, 3, 0, 4, 0, 1, q) == -4
+    assert L.magma_dgesv_rbt_batched(4, 1, 0, 4, 0, 3, 0, 1, q) == -6

@@ -201,3 +201,46 @@ def test_getrf_nopiv_zero_pivot_info(oracle):
     B = np.array([[2.0, 1.0, 0.0], [4.0, 2.0, 1.0], [1.0, 3.0, 5.0]])   # second pivot 2 - 2 = 0
     LU, info = oracle.dgetrf_nopiv(B.T.copy())
     assert info == 2
+
+
+# ------------------------------------------------------- butterfly solve ----
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 7, 8, 31, 64, 100])
+def test_rbt_transform_is_u_a_v(oracle, n):
+    """The transformed matrix factors as L U = U' A V and the solution solves
+    A X = B, with U' and V built here as explicit products of butterfly
+    matrices (each pairs x with x + N/2 and leaves an odd leftover alone)."""
+    rng = np.random.default_rng(n)
+    A = rng.uniform(-1, 1, (n, n)) + n * np.eye(n)        # dominant: no-pivot LU is accurate
+    B = rng.uniform(-1, 1, (2, n))
+    u, v = oracle.rbt_butterflies(n)
+
+    def fly(N, w, off, size):
+        M = np.eye(size)
+        h = N // 2
+        for x in range(h):
+            i, j = off + x, off + x + h
+            M[i, :] = 0; M[j, :] = 0
+            M[i, i] = w[x]; M[i, j] = w[x]
+            M[j, i] = w[x + h]; M[j, j] = -w[x + h]
+        return M
+
+    h = n // 2
+    L1 = fly(h, u[n:], 0, n) @ fly(h, u[n + h:], h, n)
+    R1 = fly(h, v[n:], 0, n) @ fly(h, v[n + h:], h, n)
+    UT = fly(n, u, 0, n) @ L1                   # level 1 first, then level 2
+    V = R1.T @ fly(n, v, 0, n).T
+    Ao, Xo, info = oracle.dgesv_rbt(np.ascontiguousarray(A.T), B)
+    LU = Ao.T
+    Lf = np.tril(LU, -1) + np.eye(n); Uf = np.triu(LU)
+    T = UT @ A @ V
+    assert info == 0
+    assert np.abs(Lf @ Uf - T).max() <= 1e-12 * max(1.0, np.abs(T).max())
+    assert oracle.solve_error(np.ascontiguousarray(A.T), Xo, B) < 30
+
+
+def test_rbt_butterflies_distribution(oracle):
+    u, v = oracle.rbt_butterflies(500)
+    for w in (u, v):
+        assert w.shape == (1000,)
+        assert np.all((w > np.exp(-0.05)) & (w < np.exp(0.05)))
+    assert not np.array_equal(u, v)

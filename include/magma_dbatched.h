@@ -67,6 +67,12 @@ magma_int_t magma_dgesv_nopiv_batched(magma_int_t n, magma_int_t nrhs, double **
                                       double **dB_array, magma_int_t lddb, magma_int_t *info_array,
                                       magma_int_t batchCount, magma_queue_t queue);
 
+/* include/magma_zbatched.h:715-721 -- Householder QR (dgeqrf's reflectors:
+ * R on and above the diagonal, the vectors below it, tau[i]); m, n <= 2048. */
+magma_int_t magma_dgeqrf_batched(magma_int_t m, magma_int_t n, double **dA_array, magma_int_t lda,
+                                 double **dtau_array, magma_int_t *info_array, magma_int_t batchCount,
+                                 magma_queue_t queue);
+
 /* include/magma_zbatched.h:609-614 -- solve by a depth-2 random butterfly
  * transformation (A <- U' A V, B <- U' B), LU without pivoting and X = V Y;
  * n <= 2048.  The butterfly weights are a fixed function of n.  Unlike the

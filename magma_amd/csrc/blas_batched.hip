@@ -447,6 +447,98 @@ void rbt_butterflies(int n, double *u, double *v)
     }
 }
 
+// Householder QR (the reflectors of netlib dgeqr2: dlarfg's beta = -sign(alpha)
+// hypot(alpha, ||x||), tau = (beta - alpha) / beta, v = (1; x / (alpha - beta));
+// dlarf applied from the left), one workgroup of 256 per matrix, in place.
+// ||x|| is max |x_r| times the root of the sum of (x_r / max)^2, so no square
+// over- or underflows.  Per column: two reductions, the reflector, w = C'v
+// with one wavefront per column of C (lanes along the rows), C -= tau v w'.
+__global__ void __launch_bounds__(256) geqrf_kernel(int m, int n, double **A_array, int lda, double **tau_array,
+                                                   int count)
+{
+    __shared__ double red[256];
+    __shared__ double coef[3];
+    extern __shared__ double dyn[];
+    double *v = dyn;              // m: the reflector (v[i] = 1)
+    double *w = dyn + m;          // n: C'v
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int k = min(m, n);
+    for (int id = blockIdx.x; id < count; id += gridDim.x) {
+        double *a = A_array[id];
+        double *tau = tau_array[id];
+#define AQ(r, c) a[(size_t)(r) + (size_t)(c) * lda]
+#pragma unroll 1
+        for (int i = 0; i < k; ++i) {
+            double mx = 0.0;
+            for (int r = i + 1 + tid; r < m; r += 256) mx = fmax(mx, fabs(AQ(r, i)));
+            red[tid] = mx;
+            __syncthreads();
+            for (int off = 128; off > 0; off >>= 1) {
+                if (tid < off) red[tid] = fmax(red[tid], red[tid + off]);
+                __syncthreads();
+            }
+            const double amax = red[0];
+            __syncthreads();
+            double sq = 0.0;
+            if (amax > 0.0)
+                for (int r = i + 1 + tid; r < m; r += 256) {
+                    const double t = AQ(r, i) / amax;
+                    sq = fma(t, t, sq);
+                }
+            red[tid] = sq;
+            __syncthreads();
+            for (int off = 128; off > 0; off >>= 1) {
+                if (tid < off) red[tid] += red[tid + off];
+                __syncthreads();
+            }
+            if (tid == 0) {
+                const double alpha = AQ(i, i);
+                const double xnorm = amax * sqrt(red[0]);
+                double t = 0.0, beta = alpha, sc = 0.0;
+                if (xnorm != 0.0) {
+                    const double xa = fabs(alpha), wmax = fmax(xa, xnorm), wmin = fmin(xa, xnorm);
+                    const double hyp = wmax * sqrt(1.0 + (wmin / wmax) * (wmin / wmax));
+                    beta = -copysign(hyp, alpha);
+                    t = (beta - alpha) / beta;
+                    sc = 1.0 / (alpha - beta);
+                }
+                coef[0] = beta;
+                coef[1] = t;
+                coef[2] = sc;
+                tau[i] = t;
+            }
+            __syncthreads();
+            const double ti = coef[1], sc = coef[2];
+            if (ti != 0.0) {
+                for (int r = i + 1 + tid; r < m; r += 256) {
+                    const double x = AQ(r, i) * sc;
+                    AQ(r, i) = x;
+                    v[r] = x;
+                }
+                if (tid == 0) {
+                    AQ(i, i) = coef[0];
+                    v[i] = 1.0;
+                }
+                __syncthreads();
+                for (int c = i + 1 + wave; c < n; c += 4) {
+                    double p = 0.0;
+                    for (int r = i + lane; r < m; r += 64) p = fma(v[r], AQ(r, c), p);
+                    for (int off = 32; off > 0; off >>= 1) p += __shfl_xor(p, off);
+                    if (lane == 0) w[c] = p;
+                }
+                __syncthreads();
+                const int rows = m - i, cols = n - i - 1;
+                for (int e = tid; e < rows * cols; e += 256) {
+                    const int r = i + e % rows, c = i + 1 + e / rows;
+                    AQ(r, c) = fma(-ti * w[c], v[r], AQ(r, c));
+                }
+            }
+            __syncthreads();
+        }
+#undef AQ
+    }
+}
+
 magma_int_t sync_unless_async(magma_queue_t queue)
 {
     if (!(queue && queue->async)) MAGMA_HIP_CHECK(hipStreamSynchronize(stream_of(queue)));
@@ -748,6 +840,33 @@ magma_int_t magma_dgesv_rbt_batched(magma_int_t n, magma_int_t nrhs, double **dA
     if (queue) queue->async = was_async;
     if (r != MAGMA_SUCCESS) return r;
     rbt_vector_kernel<<<grid, 256, 0, st>>>(n, nrhs, dB_array, lddb, dv, false, batchCount);
+    MAGMA_LAUNCH_CHECK();
+    return sync_unless_async(queue);
+}
+
+// include/magma_zbatched.h:715-721; argument checks of
+// src/zgeqrf_batched.cpp:104-115 (m -1, n -2, lda -4); m, n <= 2048.
+// info_array is set to 0 (a QR factorization always exists).
+magma_int_t magma_dgeqrf_batched(magma_int_t m, magma_int_t n, double **dA_array, magma_int_t lda,
+                                 double **dtau_array, magma_int_t *info_array, magma_int_t batchCount,
+                                 magma_queue_t queue)
+{
+    magma_int_t arg = 0;
+    if (m < 0) arg = -1;
+    else if (n < 0) arg = -2;
+    else if (lda < std::max(1, m)) arg = -4;
+    else if (batchCount < 0) arg = -7;
+    if (arg != 0) {
+        magma_xerbla(__func__, -arg);
+        return arg;
+    }
+    hipStream_t st = stream_of(queue);
+    if (batchCount == 0) return MAGMA_SUCCESS;
+    zero_info(info_array, batchCount, st);
+    if (m == 0 || n == 0) return sync_unless_async(queue);
+    if (m > 2048 || n > 2048) return MAGMA_ERR_NOT_SUPPORTED;
+    const size_t lds = sizeof(double) * (size_t)(m + n);
+    geqrf_kernel<<<persistent_grid(batchCount, 4), 256, lds, st>>>(m, n, dA_array, lda, dtau_array, batchCount);
     MAGMA_LAUNCH_CHECK();
     return sync_unless_async(queue);
 }

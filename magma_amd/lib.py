@@ -106,6 +106,7 @@ PROTOTYPES = {
     "magma_dgetrf_nopiv_batched": (c_int, [c_int, c_int, _P, c_int, _P, c_int, _P]),
     "magma_dgetrs_nopiv_batched": (c_int, [c_int, c_int, c_int, _P, c_int, _P, c_int, _P, c_int, _P]),
     "magma_dgesv_nopiv_batched": (c_int, [c_int, c_int, _P, c_int, _P, c_int, _P, c_int, _P]),
+    "magma_dgeqrf_batched": (c_int, [c_int, c_int, _P, c_int, _P, _P, c_int, _P]),
     "magma_dgesv_rbt_batched": (c_int, [c_int, c_int, _P, c_int, _P, c_int, _P, c_int, _P]),
     "magma_dgetri_outofplace_batched": (c_int, [c_int, _P, c_int, _P, _P, c_int, _P, c_int, _P]),
     "magmablas_dtrsm_batched": (None, [c_int, c_int, c_int, c_int, c_int, c_int, c_double, _P, c_int, _P, c_int,

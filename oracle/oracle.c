@@ -628,6 +628,89 @@ void orc_dgemv(char trans, int m, int n, double alpha, const double *A, int lda,
 #undef Y_
 }
 
+/* ---- Householder QR (netlib dgeqr2 with dlarfg / dlarf) --------------------
+ * Reference call site: lapackf77_zgeqrf in testing/testing_zgeqrf_batched.cpp:224;
+ * the tester checks ||R - Q'A||_1 / (n ||A||_1) and ||I - Q'Q||_1 / n
+ * (:34-82).  dnrm2 is reference BLAS's scaled sum of squares, dlapy2 the
+ * overflow-safe hypotenuse; dlarfg's rescaling of a tiny beta is kept. */
+static double nrm2(int n, const double *x)
+{
+    double scale = 0.0, ssq = 1.0;
+    for (int i = 0; i < n; ++i) {
+        if (x[i] != 0.0) {
+            const double a = fabs(x[i]);
+            if (scale < a) {
+                ssq = 1.0 + ssq * (scale / a) * (scale / a);
+                scale = a;
+            } else {
+                ssq += (a / scale) * (a / scale);
+            }
+        }
+    }
+    return scale * sqrt(ssq);
+}
+
+static double lapy2(double x, double y)
+{
+    const double xa = fabs(x), ya = fabs(y);
+    const double w = xa > ya ? xa : ya, z = xa < ya ? xa : ya;
+    if (z == 0.0) return w;
+    return w * sqrt(1.0 + (z / w) * (z / w));
+}
+
+/* dlarfg: H (alpha; x) = (beta; 0), H = I - tau v v', v = (1; x). */
+static void larfg(int n, double *alpha, double *x, double *tau)
+{
+    *tau = 0.0;
+    if (n <= 1) return;
+    double xnorm = nrm2(n - 1, x);
+    if (xnorm == 0.0) return;
+    double beta = -copysign(lapy2(*alpha, xnorm), *alpha);
+    const double safmin = DBL_MIN / (DBL_EPSILON * 0.5);
+    int knt = 0;
+    if (fabs(beta) < safmin) {
+        const double rsafmn = 1.0 / safmin;
+        do {
+            ++knt;
+            for (int i = 0; i < n - 1; ++i) x[i] *= rsafmn;
+            beta *= rsafmn;
+            *alpha *= rsafmn;
+        } while (fabs(beta) < safmin && knt < 20);
+        xnorm = nrm2(n - 1, x);
+        beta = -copysign(lapy2(*alpha, xnorm), *alpha);
+    }
+    *tau = (beta - *alpha) / beta;
+    const double sc = 1.0 / (*alpha - beta);
+    for (int i = 0; i < n - 1; ++i) x[i] *= sc;
+    for (int j = 0; j < knt; ++j) beta *= safmin;
+    *alpha = beta;
+}
+
+void orc_dgeqrf(int m, int n, double *A, int lda, double *tau)
+{
+    const int k = imin(m, n);
+    double *w = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    for (int i = 0; i < k; ++i) {
+        larfg(m - i, &A_(i, i), &A_(imin(i + 1, m - 1), i), &tau[i]);
+        if (i < n - 1 && tau[i] != 0.0) {
+            /* dlarf('Left'): w = C' v, C -= tau v w' with v = (1; A(i+1:m, i)) */
+            const double aii = A_(i, i);
+            A_(i, i) = 1.0;
+            for (int c = i + 1; c < n; ++c) {
+                double t = 0.0;
+                for (int r = i; r < m; ++r) t = fma(A_(r, c), A_(r, i), t);
+                w[c] = t;
+            }
+            for (int c = i + 1; c < n; ++c) {
+                const double t = -tau[i] * w[c];
+                for (int r = i; r < m; ++r) A_(r, c) = fma(A_(r, i), t, A_(r, c));
+            }
+            A_(i, i) = aii;
+        }
+    }
+    free(w);
+}
+
 /* LAPACK dlange: 'M' max abs, 'O'/'1' one norm, 'I' infinity norm, 'F' Frobenius. */
 double orc_dlange(char norm, int m, int n, const double *A, int lda)
 {

@@ -44,6 +44,7 @@ void orc_dpotrs(char uplo, int n, int nrhs, const double *A, int lda, double *B,
 void orc_dgetrf_nopiv(int m, int n, double *A, int lda, int *info);
 void orc_rbt_butterflies(int n, double *u, double *v);
 void orc_dgesv_rbt(int n, int nrhs, double *A, int lda, double *B, int ldb, int *info);
+void orc_dgeqrf(int m, int n, double *A, int lda, double *tau);
 void orc_dtrsm(char side, char uplo, char transa, char diag, int m, int n, double alpha,
                const double *A, int lda, double *B, int ldb);
 void orc_dtrsv(char uplo, char trans, char diag, int n, const double *A, int lda, double *x, int incx);

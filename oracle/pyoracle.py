@@ -31,6 +31,7 @@ _PROTO = {
     "orc_dpotrs": (None, [_C, _I, _I, _P, _I, _P, _I]),
     "orc_dgemm": (None, [_C, _C, _I, _I, _I, _D, _P, _I, _P, _I, _D, _P, _I]),
     "orc_dgetrf_nopiv": (None, [_I, _I, _P, _I, _P]),
+    "orc_dgeqrf": (None, [_I, _I, _P, _I, _P]),
     "orc_rbt_butterflies": (None, [_I, _P, _P]),
     "orc_dgesv_rbt": (None, [_I, _I, _P, _I, _P, _I, _P]),
     "orc_dtrsm": (None, [_C, _C, _C, _C, _I, _I, _D, _P, _I, _P, _I]),
@@ -137,6 +138,36 @@ def dgetrf_nopiv(A, m=None, n=None):
     info = np.zeros(1, dtype=np.int32)
     lib().orc_dgetrf_nopiv(m, n, _p(A), ld, _p(info))
     return A, int(info[0])
+
+
+def dgeqrf(A, m=None, n=None):
+    """Householder QR of one (cols, ld) matrix; returns (QR, tau)."""
+    A = _f64(np.array(A, dtype=np.float64, copy=True))
+    cols, ld = A.shape
+    m = ld if m is None else m
+    n = cols if n is None else n
+    tau = np.zeros(max(min(m, n), 1))
+    lib().orc_dgeqrf(m, n, _p(A), ld, _p(tau))
+    return A, tau[: min(m, n)]
+
+
+def qr_errors(A, QR, tau, m, n):
+    """The tester's two checks (testing/testing_zgeqrf_batched.cpp:34-82):
+    ||R - Q'A||_1 / (n ||A||_1) and ||I - Q'Q||_1 / n, Q formed from the
+    reflectors (numpy matrices; A and QR in (cols, ld) storage)."""
+    Am = np.asarray(A).T[:m, :n]
+    F = np.asarray(QR).T[:m, :n]
+    k = min(m, n)
+    Q = np.eye(m)
+    for i in reversed(range(k)):
+        v = np.zeros(m); v[i] = 1.0; v[i + 1:] = F[i + 1:, i]
+        Q = Q - tau[i] * np.outer(v, v @ Q)
+    Q = Q[:, :k]
+    R = np.triu(F[:k, :])
+    an = np.abs(Am).sum(axis=0).max() if Am.size else 0.0
+    e1 = np.abs(R - Q.T @ Am).sum(axis=0).max() / (n * an) if n and an else 0.0
+    e2 = np.abs(np.eye(k) - Q.T @ Q).sum(axis=0).max() / n if n else 0.0
+    return e1, e2
 
 
 def rbt_butterflies(n):

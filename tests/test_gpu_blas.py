@@ -303,3 +303,46 @@ def test_gesv_rbt_argument_errors(queue):
     assert L.magma_dgesv_rbt_batched(4, -1, 0, 4, 0, 4, 0, 1, q) == -2
     assert L.magma_dgesv_rbt_batched(4, 1, 0, 3, 0, 4, 0, 1, q) == -4
     assert L.magma_dgesv_rbt_batched(4, 1, 0, 4, 0, 3, 0, 1, q) == -6
+
+
+# -------------------------------------------------------------------- QR ----
+@pytest.mark.parametrize("m,n", [(1, 1), (2, 1), (5, 5), (32, 32), (64, 64), (100, 60), (60, 100), (256, 256),
+                                 (300, 17)])
+def test_geqrf_batched(queue, oracle, m, n):
+    """The tester's bar (testing/testing_zgeqrf_batched.cpp:34-82, 116):
+    ||R - Q'A||_1 / (n ||A||_1) < 30 eps and ||I - Q'Q||_1 / n < 30 eps, and
+    R, tau equal to the oracle's (netlib dgeqr2) to rounding.  Not bitwise:
+    the column norm is a two-pass scaled sum here, dnrm2's running scaled sum
+    of squares in the oracle."""
+    import magma_amd as ma
+    import torch
+    L = ma.load()
+    rng = np.random.default_rng(m * 1000 + n)
+    count = 5
+    A = rng.uniform(-1, 1, (count, n, m))
+    A[count - 1][0, :] = 0.0                          # a zero first column: tau[0] = 0
+    dA = ma.DeviceBatch(A, ld=m + 2)
+    k = min(m, n)
+    tau = ma.DeviceBatch(np.full((count, 1, k), 9.0))
+    info = torch.full((count,), -1, dtype=torch.int32, device="cuda")
+    print(f"geqrf m={m} n={n}", flush=True)
+    assert L.magma_dgeqrf_batched(m, n, dA.ptr, dA.ld, tau.ptr, info.data_ptr(), count, queue.handle) == 0
+    F = dA.numpy(); T = tau.numpy()[:, 0, :]
+    assert np.all(info.cpu().numpy() == 0)
+    for s in range(count):
+        e1, e2 = oracle.qr_errors(A[s], F[s], T[s], m, n)
+        assert e1 < 30 * 2.0 ** -53 and e2 < 30 * 2.0 ** -53, (s, e1, e2)
+        Fo, to = oracle.dgeqrf(A[s])
+        scale = max(1.0, np.abs(Fo).max())
+        assert np.abs(np.triu(F[s].T[:k, :]) - np.triu(Fo.T[:k, :])).max() <= 1e-12 * scale * max(m, n), s
+        assert np.abs(T[s] - to).max() <= 1e-12 * max(m, n), s
+    assert T[count - 1][0] == 0.0
+
+
+def test_geqrf_argument_errors(queue):
+    import magma_amd as ma
+    L = ma.load()
+    q = queue.handle
+    assert L.magma_dgeqrf_batched(-1, 4, 0, 4, 0, 0, 1, q) == -1
+    assert L.magma_dgeqrf_batched(4, -1, 0, 4, 0, 0, 1, q) == -2
+    assert L.magma_dgeqrf_batched(4, 4, 0, 3, 0, 0, 1, q) == -4

@@ -244,3 +244,30 @@ def test_rbt_butterflies_distribution(oracle):
         assert w.shape == (1000,)
         assert np.all((w > np.exp(-0.05)) & (w < np.exp(0.05)))
     assert not np.array_equal(u, v)
+
+
+# ------------------------------------------------------------------ QR ------
+@pytest.mark.parametrize("m,n", [(1, 1), (5, 5), (40, 40), (60, 25), (25, 60), (128, 128)])
+def test_geqrf_matches_lapack(oracle, m, n):
+    """R and tau against LAPACK's dgeqrf (scipy's OpenBLAS, third-party) to
+    rounding, and the tester's two residuals below 30 eps."""
+    from scipy.linalg import qr
+    rng = np.random.default_rng(m * 100 + n)
+    A = rng.uniform(-1, 1, (m, n))
+    (qr_raw, tau_ref), _ = qr(A, mode="raw")
+    F, tau = oracle.dgeqrf(np.ascontiguousarray(A.T))
+    k = min(m, n)
+    assert np.allclose(np.triu(F.T[:k, :]), np.triu(qr_raw[:k, :]), rtol=0, atol=1e-13 * max(m, n))
+    assert np.allclose(tau, tau_ref, rtol=0, atol=1e-13 * max(m, n))
+    e1, e2 = oracle.qr_errors(np.ascontiguousarray(A.T), F, tau, m, n)
+    assert e1 < 30 * EPS and e2 < 30 * EPS
+
+
+def test_geqrf_closed_forms(oracle):
+    # (3, 4)': beta = -5, tau = 8/5, v = (1, 1/2): all exact
+    F, tau = oracle.dgeqrf(np.array([[3.0, 4.0]]))
+    assert F[0, 0] == -5.0 and F[0, 1] == 0.5 and tau[0] == 1.6
+    # upper triangular: nothing to annihilate, tau = 0 and R = A
+    U = np.triu(np.arange(1.0, 17.0).reshape(4, 4))
+    F, tau = oracle.dgeqrf(np.ascontiguousarray(U.T))
+    assert np.all(tau == 0.0) and np.array_equal(F, U.T)

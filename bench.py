@@ -578,8 +578,89 @@ def workload_blas(args, torch, ma, L, q):
                 check=check, step_ms=step_ms, cpu=None)
 
 
+def workload_extras(args, torch, ma, L, q):
+    """magma_dgeqrf_batched, magma_dgetri_outofplace_batched (from the getrf
+    factors, computed before the timed region), magma_dgesv_nopiv_batched and
+    magma_dgesv_rbt_batched (one right-hand side) on 20,000 matrices of order
+    64, each timed on its own; one step runs the four calls."""
+    count, n = 20000, 64
+    rng = np.random.default_rng(6)
+    A0 = rng.uniform(-1, 1, (count, n, n))
+    A0[:, np.arange(n), np.arange(n)] += n           # no-pivot LU stays accurate
+    A0 = torch.from_numpy(A0).cuda()
+    B0 = torch.from_numpy(rng.uniform(-1, 1, (count, 1, n))).cuda()
+    LU = A0.clone()
+    p_lu = ma.ptr_array(LU, count, n * n)
+    ipiv = torch.zeros((count, n), dtype=torch.int32, device="cuda")
+    p_ipiv = ma.ptr_array(ipiv, count, n)
+    info = torch.zeros(count, dtype=torch.int32, device="cuda")
+    L.magma_dgetrf_batched(n, n, p_lu.data_ptr(), n, p_ipiv.data_ptr(), info.data_ptr(), count, q.handle)
+    torch.cuda.synchronize()
+    names = ("geqrf", "getri", "gesv_nopiv", "gesv_rbt")
+    timers = {k: Timer(torch) for k in names}
+
+    def make(i):
+        Aq = A0.clone(); tau = torch.zeros((count, n), dtype=torch.float64, device="cuda")
+        inv = torch.zeros_like(A0)
+        An = A0.clone(); Bn = B0.clone(); Ar = A0.clone(); Br = B0.clone()
+        pp = [ma.ptr_array(t, count, t[0].numel()) for t in (Aq, tau, inv, An, Bn, Ar, Br)]
+        return (Aq, tau, inv, An, Bn, Ar, Br), pp
+
+    def step(c, timer):
+        _, (pq, pt, pinv, pan, pbn, par, pbr) = c
+        for name in names:
+            e = timers[name].start() if timer else None
+            if name == "geqrf":
+                L.magma_dgeqrf_batched(n, n, pq.data_ptr(), n, pt.data_ptr(), info.data_ptr(), count, q.handle)
+            elif name == "getri":
+                L.magma_dgetri_outofplace_batched(n, p_lu.data_ptr(), n, p_ipiv.data_ptr(), pinv.data_ptr(), n,
+                                                  info.data_ptr(), count, q.handle)
+            elif name == "gesv_nopiv":
+                L.magma_dgesv_nopiv_batched(n, 1, pan.data_ptr(), n, pbn.data_ptr(), n, info.data_ptr(), count,
+                                            q.handle)
+            else:
+                L.magma_dgesv_rbt_batched(n, 1, par.data_ptr(), n, pbr.data_ptr(), n, info.data_ptr(), count,
+                                          q.handle)
+            if timer:
+                timers[name].stop(e)
+
+    wall, step_ms, copies = run_fixed(args, torch, ma, L, q, make, step, count, count, "matrices")
+    check = {}
+    if args.check:
+        from oracle import pyoracle
+        (Aq, tau, inv, An, Bn, Ar, Br), _ = copies[-1]
+        A_h = A0[:8].cpu().numpy(); B_h = B0[:8].cpu().numpy()
+        ok = True
+        worst = 0.0
+        for s in range(8):
+            e1, e2 = pyoracle.qr_errors(A_h[s], Aq[s].cpu().numpy(), tau[s].cpu().numpy(), n, n)
+            worst = max(worst, e1 / pyoracle.EPS, e2 / pyoracle.EPS)
+            _, Xo, _ = pyoracle.dgesv_rbt(A_h[s], B_h[s])
+            ok &= np.array_equal(Br[s].cpu().numpy(), Xo)
+            LUo, info_o = pyoracle.dgetrf_nopiv(A_h[s])
+            ok &= np.array_equal(An[s].cpu().numpy(), LUo)
+        check = {"sample": 8, "qr_max_residual_over_eps": float(worst), "nopiv_rbt_bitwise_equal_to_oracle": bool(ok)}
+    ms = wall / args.steps * 1e3
+    kms = {k: float(np.mean(t.ms())) for k, t in timers.items()}
+    # algorithmic bytes per matrix: the operands read and written once
+    byts = {"geqrf": count * (16.0 * n * n + 8.0 * n), "getri": count * (8.0 * n * n + 4.0 * n + 8.0 * n * n),
+            "gesv_nopiv": count * (16.0 * n * n + 16.0 * n), "gesv_rbt": count * (16.0 * n * n + 16.0 * n)}
+    rates = {k: byts[k] / (kms[k] * 1e-3) / 1e9 for k in kms}
+    flops = {"geqrf": count * (4.0 / 3.0) * n ** 3}
+    return dict(value=count / (ms * 1e-3), unit="matrices/s (geqrf + getri + gesv_nopiv + gesv_rbt)", ms=ms,
+                dtype="f64",
+                config={"workload": "extras_batched", "batch": count, "n": n, "nrhs": 1, "kernel_ms": kms,
+                        "algorithmic_GBps": rates,
+                        "geqrf_tflops": flops["geqrf"] / (kms["geqrf"] * 1e-3) / 1e12,
+                        "inputs": "A U[-1,1] + 64 I (seed 6), b U[-1,1]"},
+                roofline={"bound": "hbm", "achieved": rates["geqrf"], "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                          "kernel": "geqrf_kernel", "kernel_ms": kms["geqrf"], "bytes_per_launch": byts["geqrf"]},
+                check=check, step_ms=step_ms, cpu=None)
+
+
 WORKLOADS = {
     "cond256_mixed_vs_plain": workload_cond256,
+    "extras_batched": workload_extras,
     "blas_batched": workload_blas,
     "lu32_gesv": workload_lu32,
     "chol64_posv": workload_chol64,

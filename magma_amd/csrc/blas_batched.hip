@@ -33,6 +33,7 @@
 //         column in row order.
 #include <algorithm>
 #include <cfloat>
+#include <climits>
 #include <cmath>
 #include <cstdint>
 
@@ -456,7 +457,7 @@ void rbt_butterflies(int n, double *u, double *v)
 __global__ void __launch_bounds__(256) geqrf_kernel(int m, int n, double **A_array, int lda, double **tau_array,
                                                    int count)
 {
-    __shared__ double red[256];
+    __shared__ double redm[4], reds[4];
     __shared__ double coef[3];
     extern __shared__ double dyn[];
     double *v = dyn;              // m: the reflector (v[i] = 1)
@@ -469,31 +470,26 @@ __global__ void __launch_bounds__(256) geqrf_kernel(int m, int n, double **A_arr
 #define AQ(r, c) a[(size_t)(r) + (size_t)(c) * lda]
 #pragma unroll 1
         for (int i = 0; i < k; ++i) {
+            // max |x| and sum (x / max)^2: wavefront shuffles, then the four
+            // wavefronts' values through LDS
             double mx = 0.0;
             for (int r = i + 1 + tid; r < m; r += 256) mx = fmax(mx, fabs(AQ(r, i)));
-            red[tid] = mx;
+            for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+            if (lane == 0) redm[wave] = mx;
             __syncthreads();
-            for (int off = 128; off > 0; off >>= 1) {
-                if (tid < off) red[tid] = fmax(red[tid], red[tid + off]);
-                __syncthreads();
-            }
-            const double amax = red[0];
-            __syncthreads();
+            const double amax = fmax(fmax(redm[0], redm[1]), fmax(redm[2], redm[3]));
             double sq = 0.0;
             if (amax > 0.0)
                 for (int r = i + 1 + tid; r < m; r += 256) {
                     const double t = AQ(r, i) / amax;
                     sq = fma(t, t, sq);
                 }
-            red[tid] = sq;
+            for (int off = 32; off > 0; off >>= 1) sq += __shfl_xor(sq, off);
+            if (lane == 0) reds[wave] = sq;
             __syncthreads();
-            for (int off = 128; off > 0; off >>= 1) {
-                if (tid < off) red[tid] += red[tid + off];
-                __syncthreads();
-            }
             if (tid == 0) {
                 const double alpha = AQ(i, i);
-                const double xnorm = amax * sqrt(red[0]);
+                const double xnorm = amax * sqrt((reds[0] + reds[1]) + (reds[2] + reds[3]));
                 double t = 0.0, beta = alpha, sc = 0.0;
                 if (xnorm != 0.0) {
                     const double xa = fabs(alpha), wmax = fmax(xa, xnorm), wmin = fmin(xa, xnorm);
@@ -536,6 +532,44 @@ __global__ void __launch_bounds__(256) geqrf_kernel(int m, int n, double **A_arr
             __syncthreads();
         }
 #undef AQ
+    }
+}
+
+// invA <- P' I, the identity with getrf's row interchanges applied in order
+// (row i is e_perm(i)', perm tracked in LDS), and info = the first k (1-based)
+// with U(k,k) exactly zero, else 0.
+__global__ void __launch_bounds__(256) perm_identity_kernel(int n, double **A_array, int lda, int **ipiv_array,
+                                                           double **I_array, int ldi, int *info, int count)
+{
+    extern __shared__ int perm[];
+    __shared__ int first;
+    const int tid = threadIdx.x;
+    for (int id = blockIdx.x; id < count; id += gridDim.x) {
+        const int *ipiv = ipiv_array[id];
+        if (tid == 0) {
+            first = INT_MAX;
+            for (int i = 0; i < n; ++i) perm[i] = i;
+            for (int k = 0; k < n; ++k) {
+                const int p = ipiv[k] - 1;
+                if (p >= 0 && p < n && p != k) {
+                    const int t = perm[k];
+                    perm[k] = perm[p];
+                    perm[p] = t;
+                }
+            }
+        }
+        __syncthreads();
+        const double *a = A_array[id];
+        for (int i = tid; i < n; i += 256)
+            if (a[i + (size_t)i * lda] == 0.0) atomicMin(&first, i + 1);
+        double *w = I_array[id];
+        for (int e = tid; e < n * n; e += 256) {
+            const int i = e % n, j = e / n;
+            w[i + (size_t)j * ldi] = (j == perm[i]) ? 1.0 : 0.0;
+        }
+        __syncthreads();
+        if (tid == 0 && info) info[id] = (first == INT_MAX) ? 0 : first;
+        __syncthreads();
     }
 }
 
@@ -868,6 +902,44 @@ magma_int_t magma_dgeqrf_batched(magma_int_t m, magma_int_t n, double **dA_array
     const size_t lds = sizeof(double) * (size_t)(m + n);
     geqrf_kernel<<<persistent_grid(batchCount, 4), 256, lds, st>>>(m, n, dA_array, lda, dtau_array, batchCount);
     MAGMA_LAUNCH_CHECK();
+    return sync_unless_async(queue);
+}
+
+// src/zgetri_outofplace_batched.cpp:82-220.  The reference sets invA = I,
+// solves with L (unit) and U, then interchanges the columns in reverse pivot
+// order; column j of that is the solution of L U x = e_p(j).  Here invA = P'I
+// (row i is e_perm(i)', so column j is e_p(j)) and the two triangular solves
+// run on it: the same arithmetic per column as getrs applied to the identity,
+// which is bitwise what the reference's sequence computes.  info[i] = k > 0
+// reports U(k,k) == 0 (LAPACK dgetri's code); the inverse is still formed, as
+// the reference does.
+magma_int_t magma_dgetri_outofplace_batched(magma_int_t n, double **dA_array, magma_int_t ldda,
+                                            magma_int_t **dipiv_array, double **dinvA_array, magma_int_t lddia,
+                                            magma_int_t *info_array, magma_int_t batchCount, magma_queue_t queue)
+{
+    magma_int_t arg = 0;
+    if (n < 0) arg = -1;
+    else if (ldda < std::max(1, n)) arg = -3;
+    else if (lddia < std::max(1, n)) arg = -6;
+    else if (batchCount < 0) arg = -8;
+    if (arg != 0) {
+        magma_xerbla(__func__, -arg);
+        return arg;
+    }
+    hipStream_t st = stream_of(queue);
+    if (batchCount == 0) return MAGMA_SUCCESS;
+    if (n == 0) {
+        zero_info(info_array, batchCount, st);
+        return sync_unless_async(queue);
+    }
+    if (n > 2048) return MAGMA_ERR_NOT_SUPPORTED;
+    perm_identity_kernel<<<persistent_grid(batchCount, 4), 256, sizeof(int) * (size_t)n, st>>>(
+        n, dA_array, ldda, dipiv_array, dinvA_array, lddia, info_array, batchCount);
+    MAGMA_LAUNCH_CHECK();
+    magma_int_t r = launch_trsm<'L', 'L', 'N'>(n, n, 1.0, dA_array, ldda, dinvA_array, lddia, true, batchCount, st);
+    if (r == MAGMA_SUCCESS)
+        r = launch_trsm<'L', 'U', 'N'>(n, n, 1.0, dA_array, ldda, dinvA_array, lddia, false, batchCount, st);
+    if (r != MAGMA_SUCCESS) return r;
     return sync_unless_async(queue);
 }
 

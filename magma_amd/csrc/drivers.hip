@@ -28,28 +28,6 @@ __global__ void fill_int_kernel(int *p, int v, int count)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) p[i] = v;
 }
-
-// invA <- I, and info = the first i (1-based) with U(i,i) exactly zero, else 0.
-__global__ void __launch_bounds__(256) identity_info_kernel(int n, double **A_array, int lda, double **I_array,
-                                                           int ldi, int *info, int count)
-{
-    __shared__ int first;
-    for (int id = blockIdx.x; id < count; id += gridDim.x) {
-        if (threadIdx.x == 0) first = INT_MAX;
-        __syncthreads();
-        const double *a = A_array[id];
-        double *w = I_array[id];
-        for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
-            const int i = e % n, j = e / n;
-            w[i + (size_t)j * ldi] = (i == j) ? 1.0 : 0.0;
-        }
-        for (int i = threadIdx.x; i < n; i += blockDim.x)
-            if (a[i + (size_t)i * lda] == 0.0) atomicMin(&first, i + 1);
-        __syncthreads();
-        if (threadIdx.x == 0 && info) info[id] = (first == INT_MAX) ? 0 : first;
-        __syncthreads();
-    }
-}
 }  // namespace
 
 // Sets info_array[0..count) to zero with a kernel (safe under stream capture).
@@ -259,44 +237,6 @@ magma_int_t magma_dposv_batched(magma_uplo_t uplo, magma_int_t n, magma_int_t nr
     s.B = dB_array; s.nrhs = nrhs; s.ldb = lddb;
     s.count = batchCount;
     return finish(launch_potrf(s, n, true, st), queue);
-}
-
-// src/zgetri_outofplace_batched.cpp:82-220.  The reference sets invA = I,
-// solves with L (unit) and U per block of columns, then interchanges the
-// columns in reverse pivot order.  Column j of that result is the solution of
-// L U x = e_p(j), so the same matrix comes out, with the same arithmetic per
-// column, from the plain solve (getrs, NoTrans) applied to the identity,
-// whose row interchanges place each e_p(j) in column j.  info[i] = k > 0
-// reports U(k,k) == 0 (LAPACK dgetri's code); the inverse is still formed,
-// as the reference does.
-magma_int_t magma_dgetri_outofplace_batched(magma_int_t n, double **dA_array, magma_int_t ldda,
-                                            magma_int_t **dipiv_array, double **dinvA_array, magma_int_t lddia,
-                                            magma_int_t *info_array, magma_int_t batchCount, magma_queue_t queue)
-{
-    magma_int_t info = 0;
-    if (n < 0) info = -1;
-    else if (ldda < std::max(1, n)) info = -3;
-    else if (lddia < std::max(1, n)) info = -6;
-    else if (batchCount < 0) info = -8;
-    if (info != 0) {
-        magma_xerbla(__func__, -info);
-        return info;
-    }
-    hipStream_t st = stream_of(queue);
-    if (n == 0 || batchCount == 0) {
-        zero_info(info_array, batchCount, st);
-        return finish(MAGMA_SUCCESS, queue);
-    }
-    if (n > 2048) return MAGMA_ERR_NOT_SUPPORTED;
-    identity_info_kernel<<<persistent_grid(batchCount, 4), 256, 0, st>>>(n, dA_array, ldda, dinvA_array, lddia,
-                                                                        info_array, batchCount);
-    MAGMA_LAUNCH_CHECK();
-    MatSet s;
-    s.m = n; s.n = n; s.lda = ldda;
-    s.A = dA_array; s.ipiv = dipiv_array;
-    s.B = dinvA_array; s.nrhs = n; s.ldb = lddia;
-    s.count = batchCount;
-    return finish(launch_getrs(s, 'N', n, n, st), queue);
 }
 
 }  // extern "C"

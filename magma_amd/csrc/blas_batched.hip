@@ -45,6 +45,9 @@ magma_int_t zero_info(int *info, int count, hipStream_t st);
 
 namespace {
 
+// Largest order the batched trsv takes (x and one pivot of A in 64 KB of LDS).
+#define MAGMA_AMD_TRSV_MAX_N 4095
+
 constexpr int kTrsvThreads = 256;
 constexpr int kTrsmStagedMax = 128;        // vectors up to this length live in LDS (64 KB per wavefront)
 constexpr size_t kTrsvLds = 64 * 1024;     // x plus the staged panel of A
@@ -70,18 +73,21 @@ __global__ void __launch_bounds__(NT) trsv_kernel(int n, double **A_array, int l
             // stage the panel: pivot p_c = step s0 + c; NoTrans column p_c,
             // Trans row p_c, all n entries (c fastest for Trans, so each
             // column of A is read as one contiguous run of pivots' rows)
+            constexpr int W = NT / 64;
+            const int lane = tid & 63, wv = tid >> 6;
             if (!TRANS) {
-                for (int e = tid; e < np * n; e += NT) {
-                    const int c = e / n, i = e % n;
+#pragma unroll 8
+                for (int c = wv; c < np; c += W) {
                     const int p = asc ? s0 + c : n - 1 - (s0 + c);
-                    pan[c * n + i] = a[i + (size_t)p * lda];
+                    for (int i = lane; i < n; i += 64) pan[c * n + i] = a[i + (size_t)p * lda];
                 }
             } else {
-                for (int e = tid; e < np * n; e += NT) {
-                    const int c = e % np, j = e / np;
-                    const int p = asc ? s0 + c : n - 1 - (s0 + c);
-                    pan[c * n + j] = a[p + (size_t)j * lda];
-                }
+#pragma unroll 8
+                for (int j = wv; j < n; j += W)
+                    for (int c = lane; c < np; c += 64) {
+                        const int p = asc ? s0 + c : n - 1 - (s0 + c);
+                        pan[c * n + j] = a[p + (size_t)j * lda];
+                    }
             }
             __syncthreads();
 #pragma unroll 1
@@ -140,11 +146,14 @@ __global__ void __launch_bounds__(64) trsm_kernel(int m, int n, double alpha, do
             for (int k = 0; k < L; ++k) GV(k) = 0.0;
             continue;
         }
-        if (STAGED)
+        if (STAGED) {
+#pragma unroll 8
             for (int k = 0; k < L; ++k) vec[(size_t)k * 64 + lane] = GV(k);
+        }
         if (SIDE == 'L') {
             if (TRANS == 'N') {
                 if (alpha != 1.0)
+                    #pragma unroll 4
                     for (int i = 0; i < m; ++i) XV(i) = alpha * XV(i);
                 if (UPLO == 'U') {
                     for (int k = m - 1; k >= 0; --k) {
@@ -152,6 +161,7 @@ __global__ void __launch_bounds__(64) trsm_kernel(int m, int n, double alpha, do
                         if (bk != 0.0) {
                             if (!unit) bk = bk / AA(k, k);
                             XV(k) = bk;
+                            #pragma unroll 4
                             for (int i = 0; i < k; ++i) XV(i) = fma(-bk, AA(i, k), XV(i));
                         }
                     }
@@ -161,6 +171,7 @@ __global__ void __launch_bounds__(64) trsm_kernel(int m, int n, double alpha, do
                         if (bk != 0.0) {
                             if (!unit) bk = bk / AA(k, k);
                             XV(k) = bk;
+                            #pragma unroll 4
                             for (int i = k + 1; i < m; ++i) XV(i) = fma(-bk, AA(i, k), XV(i));
                         }
                     }
@@ -168,6 +179,7 @@ __global__ void __launch_bounds__(64) trsm_kernel(int m, int n, double alpha, do
             } else if (UPLO == 'U') {
                 for (int i = 0; i < m; ++i) {
                     double t = alpha * XV(i);
+                    #pragma unroll 4
                     for (int k = 0; k < i; ++k) t = fma(-AA(k, i), XV(k), t);
                     if (!unit) t = t / AA(i, i);
                     XV(i) = t;
@@ -175,6 +187,7 @@ __global__ void __launch_bounds__(64) trsm_kernel(int m, int n, double alpha, do
             } else {
                 for (int i = m - 1; i >= 0; --i) {
                     double t = alpha * XV(i);
+                    #pragma unroll 4
                     for (int k = i + 1; k < m; ++k) t = fma(-AA(k, i), XV(k), t);
                     if (!unit) t = t / AA(i, i);
                     XV(i) = t;
@@ -314,7 +327,9 @@ __global__ void __launch_bounds__(256) getrf_nopiv_kernel(int m, int n, double *
         double *a = LDSM ? sa : ga;
         const int ld = LDSM ? m : lda;
         if (LDSM) {
-            for (int e = tid; e < m * n; e += 256) sa[e] = ga[(size_t)(e % m) + (size_t)(e / m) * lda];
+#pragma unroll 8
+            for (int c = tid >> 6; c < n; c += 4)
+                for (int i = tid & 63; i < m; i += 64) sa[i + c * m] = ga[i + (size_t)c * lda];
             __syncthreads();
         }
         int inf = 0;
@@ -333,17 +348,20 @@ __global__ void __launch_bounds__(256) getrf_nopiv_kernel(int m, int n, double *
             }
             __syncthreads();
             if (j < mn - 1) {
-                const int rows = m - j - 1, cols = n - j - 1;
-                for (int e = tid; e < rows * cols; e += 256) {
-                    const int i = j + 1 + e % rows, c = j + 1 + e / rows;
+                // lanes along the rows, the four wavefronts along the columns
+#pragma unroll 4
+                for (int c = j + 1 + (tid >> 6); c < n; c += 4) {
                     const double y = a[j + (size_t)c * ld];
-                    if (y != 0.0) a[i + (size_t)c * ld] = fma(a[i + (size_t)j * ld], -y, a[i + (size_t)c * ld]);
+                    if (y != 0.0)
+                        for (int i = j + 1 + (tid & 63); i < m; i += 64)
+                            a[i + (size_t)c * ld] = fma(a[i + (size_t)j * ld], -y, a[i + (size_t)c * ld]);
                 }
             }
             __syncthreads();
         }
         if (LDSM)
-            for (int e = tid; e < m * n; e += 256) ga[(size_t)(e % m) + (size_t)(e / m) * lda] = sa[e];
+            for (int c = tid >> 6; c < n; c += 4)
+                for (int i = tid & 63; i < m; i += 64) ga[i + (size_t)c * lda] = sa[i + c * m];
         if (tid == 0 && info) info[id] = inf;
         __syncthreads();
     }
@@ -370,20 +388,30 @@ __global__ void __launch_bounds__(256) rbt_matrix_kernel(int n, double **A_array
             const bool rows = (pass & 1) == 0, lvl1 = pass < 2;
             const int pairs = lvl1 ? hh : h;          // pairs per block
             const int blocks = lvl1 ? 2 : 1;
-            const int items = blocks * pairs * n;
-            for (int e = tid; e < items; e += 256) {
-                // the paired index runs fastest: rows passes read down a column
-                const int x = e % pairs, rest = e / pairs;
-                const int other = rest % n, b = rest / n;
+            const int N2 = lvl1 ? hh : h;             // pair distance
+            const int lane = tid & 63, wv = tid >> 6;
+            for (int b = 0; b < blocks; ++b) {
                 const int o = lvl1 ? b * h : 0;
-                const int N2 = lvl1 ? hh : h;         // pair distance
                 const double *w = lvl1 ? ((rows ? u : v) + n + b * h) : (rows ? u : v);
-                const int i0 = o + x, i1 = o + x + N2;
-                double *p0 = rows ? &AE(i0, other) : &AE(other, i0);
-                double *p1 = rows ? &AE(i1, other) : &AE(other, i1);
-                const double s0 = *p0, s1 = *p1;
-                *p0 = w[x] * (s0 + s1);
-                *p1 = w[x + N2] * (s0 - s1);
+                // lanes run along the contiguous index: the pair index for a
+                // rows pass (down a column), the row for a columns pass
+                if (rows) {
+                    for (int c = wv; c < n; c += 4)
+                        for (int x = lane; x < pairs; x += 64) {
+                            double *p0 = &AE(o + x, c), *p1 = &AE(o + x + N2, c);
+                            const double s0 = *p0, s1 = *p1;
+                            *p0 = w[x] * (s0 + s1);
+                            *p1 = w[x + N2] * (s0 - s1);
+                        }
+                } else {
+                    for (int x = wv; x < pairs; x += 4)
+                        for (int r = lane; r < n; r += 64) {
+                            double *p0 = &AE(r, o + x), *p1 = &AE(r, o + x + N2);
+                            const double s0 = *p0, s1 = *p1;
+                            *p0 = w[x] * (s0 + s1);
+                            *p1 = w[x + N2] * (s0 - s1);
+                        }
+                }
             }
             __syncthreads();
         }
@@ -404,22 +432,24 @@ __global__ void __launch_bounds__(256) rbt_vector_kernel(int n, int nrhs, double
             const bool lvl1 = forward ? step == 0 : step == 1;
             const int pairs = lvl1 ? hh : h;
             const int blocks = lvl1 ? 2 : 1;
-            const int items = blocks * pairs * nrhs;
-            for (int e = tid; e < items; e += 256) {
-                const int x = e % pairs, rest = e / pairs;
-                const int j = rest % nrhs, blk = rest / nrhs;
+            const int N2 = lvl1 ? hh : h;
+            const int lane = tid & 63, wv = tid >> 6;
+            for (int blk = 0; blk < blocks; ++blk) {
                 const int o = lvl1 ? blk * h : 0;
-                const int N2 = lvl1 ? hh : h;
                 const double *ww = lvl1 ? w + n + blk * h : w;
-                double *b = bm + (size_t)j * ldb + o;
-                if (forward) {
-                    const double a1 = b[x] + b[x + N2], a2 = b[x] - b[x + N2];
-                    b[x] = ww[x] * a1;
-                    b[x + N2] = ww[x + N2] * a2;
-                } else {
-                    const double a1 = ww[x] * b[x], a2 = ww[x + N2] * b[x + N2];
-                    b[x] = a1 + a2;
-                    b[x + N2] = a1 - a2;
+                for (int j = wv; j < nrhs; j += 4) {
+                    double *b = bm + (size_t)j * ldb + o;
+                    for (int x = lane; x < pairs; x += 64) {
+                        if (forward) {
+                            const double a1 = b[x] + b[x + N2], a2 = b[x] - b[x + N2];
+                            b[x] = ww[x] * a1;
+                            b[x + N2] = ww[x + N2] * a2;
+                        } else {
+                            const double a1 = ww[x] * b[x], a2 = ww[x + N2] * b[x + N2];
+                            b[x] = a1 + a2;
+                            b[x + N2] = a1 - a2;
+                        }
+                    }
                 }
             }
             __syncthreads();
@@ -516,6 +546,7 @@ __global__ void __launch_bounds__(256) geqrf_kernel(int m, int n, double **A_arr
                     v[i] = 1.0;
                 }
                 __syncthreads();
+#pragma unroll 4
                 for (int c = i + 1 + wave; c < n; c += 4) {
                     double p = 0.0;
                     for (int r = i + lane; r < m; r += 64) p = fma(v[r], AQ(r, c), p);
@@ -523,10 +554,10 @@ __global__ void __launch_bounds__(256) geqrf_kernel(int m, int n, double **A_arr
                     if (lane == 0) w[c] = p;
                 }
                 __syncthreads();
-                const int rows = m - i, cols = n - i - 1;
-                for (int e = tid; e < rows * cols; e += 256) {
-                    const int r = i + e % rows, c = i + 1 + e / rows;
-                    AQ(r, c) = fma(-ti * w[c], v[r], AQ(r, c));
+#pragma unroll 4
+                for (int c = i + 1 + wave; c < n; c += 4) {
+                    const double t = -ti * w[c];
+                    for (int r = i + lane; r < m; r += 64) AQ(r, c) = fma(t, v[r], AQ(r, c));
                 }
             }
             __syncthreads();
@@ -563,10 +594,8 @@ __global__ void __launch_bounds__(256) perm_identity_kernel(int n, double **A_ar
         for (int i = tid; i < n; i += 256)
             if (a[i + (size_t)i * lda] == 0.0) atomicMin(&first, i + 1);
         double *w = I_array[id];
-        for (int e = tid; e < n * n; e += 256) {
-            const int i = e % n, j = e / n;
-            w[i + (size_t)j * ldi] = (j == perm[i]) ? 1.0 : 0.0;
-        }
+        for (int j = tid >> 6; j < n; j += 4)
+            for (int i = tid & 63; i < n; i += 64) w[i + (size_t)j * ldi] = (j == perm[i]) ? 1.0 : 0.0;
         __syncthreads();
         if (tid == 0 && info) info[id] = (first == INT_MAX) ? 0 : first;
         __syncthreads();
@@ -601,16 +630,21 @@ magma_int_t launch_trsm(int m, int n, double alpha, double **A, int lda, double 
                         hipStream_t st)
 {
     const int L = S == 'L' ? m : n, nv = S == 'L' ? n : m;
+    // one right-hand side, side Left, alpha 1: dtrsm's loops for NoTrans (both
+    // triangles) and for Trans/Upper are dtrsv's, so the workgroup-per-matrix
+    // trsv kernel gives the same bits with the pivots' columns staged in LDS
+    if (S == 'L' && n == 1 && alpha == 1.0 && (T == 'N' || U == 'U') && m <= MAGMA_AMD_TRSV_MAX_N)
+        return launch_trsv<T == 'T', U == 'U'>(m, A, lda, B, 1, unit, count, st);
     int G = 1;
     while (G < 64 && G < nv) G *= 2;
     const long items = (long)((count + 64 / G - 1) / (64 / G)) * ((nv + G - 1) / G);
     if (L <= kTrsmStagedMax) {
         const size_t lds = sizeof(double) * 64 * (size_t)L;
-        trsm_kernel<S, U, T, true><<<persistent_grid(items, 8), 64, lds, st>>>(m, n, alpha, A, lda, B, ldb, unit,
-                                                                                count, G);
+        trsm_kernel<S, U, T, true><<<persistent_grid(items, 8), 64, lds, st>>>(m, n, alpha, A, lda, B, ldb,
+                                                                                       unit, count, G);
     } else {
-        trsm_kernel<S, U, T, false><<<persistent_grid(items, 8), 64, 0, st>>>(m, n, alpha, A, lda, B, ldb, unit,
-                                                                               count, G);
+        trsm_kernel<S, U, T, false><<<persistent_grid(items, 8), 64, 0, st>>>(m, n, alpha, A, lda, B, ldb,
+                                                                                      unit, count, G);
     }
     MAGMA_LAUNCH_CHECK();
     return MAGMA_SUCCESS;
@@ -623,8 +657,6 @@ using namespace magma_amd;
 
 extern "C" {
 
-// Largest order the batched trsv takes (x and one pivot of A in 64 KB of LDS).
-#define MAGMA_AMD_TRSV_MAX_N 4095
 
 // include/magma_zbatched.h:237-242.  Argument checks and codes of
 // magmablas/ztrsv_batched.cu:82-92, which report n as -5 and lda as -8;

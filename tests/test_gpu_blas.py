@@ -59,7 +59,8 @@ def test_trsm_batched_bitexact(queue, oracle, side, uplo, trans, diag):
     L = ma.load()
     rng = np.random.default_rng(TRSM_CASES.index((side, uplo, trans, diag)))
     es, eu, et, ed = enums(side, uplo, trans, diag)
-    for (m, n), alpha in itertools.product([(1, 1), (7, 5), (33, 17), (64, 70), (100, 3)], [1.0, 0.7, 0.0]):
+    for (m, n), alpha in itertools.product([(1, 1), (7, 5), (33, 17), (64, 70), (100, 3), (40, 1), (64, 64)],
+                                           [1.0, 0.7, 0.0]):
         count = 5
         k = m if side == "L" else n
         A = tri_batch(rng, count, k, uplo, diag)

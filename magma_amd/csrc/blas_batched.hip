@@ -484,6 +484,7 @@ void rbt_butterflies(int n, double *u, double *v)
 // ||x|| is max |x_r| times the root of the sum of (x_r / max)^2, so no square
 // over- or underflows.  Per column: two reductions, the reflector, w = C'v
 // with one wavefront per column of C (lanes along the rows), C -= tau v w'.
+template <bool LDSM>
 __global__ void __launch_bounds__(256) geqrf_kernel(int m, int n, double **A_array, int lda, double **tau_array,
                                                    int count)
 {
@@ -495,9 +496,18 @@ __global__ void __launch_bounds__(256) geqrf_kernel(int m, int n, double **A_arr
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int k = min(m, n);
     for (int id = blockIdx.x; id < count; id += gridDim.x) {
-        double *a = A_array[id];
+        double *ga = A_array[id];
         double *tau = tau_array[id];
-#define AQ(r, c) a[(size_t)(r) + (size_t)(c) * lda]
+        // LDSM: the matrix is held in LDS (after v and w) for the whole factorization
+        double *a = LDSM ? dyn + m + n : ga;
+        const int ld = LDSM ? m : lda;
+        if (LDSM) {
+#pragma unroll 8
+            for (int c = wave; c < n; c += 4)
+                for (int r = lane; r < m; r += 64) a[r + c * m] = ga[r + (size_t)c * lda];
+            __syncthreads();
+        }
+#define AQ(r, c) a[(size_t)(r) + (size_t)(c) * ld]
 #pragma unroll 1
         for (int i = 0; i < k; ++i) {
             // max |x| and sum (x / max)^2: wavefront shuffles, then the four
@@ -560,6 +570,11 @@ __global__ void __launch_bounds__(256) geqrf_kernel(int m, int n, double **A_arr
                     for (int r = i + lane; r < m; r += 64) AQ(r, c) = fma(t, v[r], AQ(r, c));
                 }
             }
+            __syncthreads();
+        }
+        if (LDSM) {
+            for (int c = wave; c < n; c += 4)
+                for (int r = lane; r < m; r += 64) ga[r + (size_t)c * lda] = a[r + c * m];
             __syncthreads();
         }
 #undef AQ
@@ -931,8 +946,16 @@ magma_int_t magma_dgeqrf_batched(magma_int_t m, magma_int_t n, double **dA_array
     zero_info(info_array, batchCount, st);
     if (m == 0 || n == 0) return sync_unless_async(queue);
     if (m > 2048 || n > 2048) return MAGMA_ERR_NOT_SUPPORTED;
-    const size_t lds = sizeof(double) * (size_t)(m + n);
-    geqrf_kernel<<<persistent_grid(batchCount, 4), 256, lds, st>>>(m, n, dA_array, lda, dtau_array, batchCount);
+    const size_t vw = sizeof(double) * (size_t)(m + n);
+    const size_t whole = vw + sizeof(double) * (size_t)m * n;
+    if (whole <= 64 * 1024) {
+        const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, (160 * 1024) / whole));
+        geqrf_kernel<true><<<persistent_grid(batchCount, per_cu), 256, whole, st>>>(m, n, dA_array, lda, dtau_array,
+                                                                                    batchCount);
+    } else {
+        geqrf_kernel<false><<<persistent_grid(batchCount, 4), 256, vw, st>>>(m, n, dA_array, lda, dtau_array,
+                                                                             batchCount);
+    }
     MAGMA_LAUNCH_CHECK();
     return sync_unless_async(queue);
 }

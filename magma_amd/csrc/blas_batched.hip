@@ -665,6 +665,39 @@ magma_int_t launch_trsm(int m, int n, double alpha, double **A, int lda, double 
     return MAGMA_SUCCESS;
 }
 
+// Enqueue only (no argument checks, no synchronization): the no-pivot
+// factorization, and the two triangular solves of getrs without interchanges
+// (reference dtrsm order, as dgetrs does).
+magma_int_t enqueue_getrf_nopiv(int m, int n, double **A, int lda, int *info, int count, hipStream_t st)
+{
+    if (m == 0 || n == 0) return zero_info(info, count, st);
+    if (m > 2048 || n > 2048) return MAGMA_ERR_NOT_SUPPORTED;
+    const size_t lds = sizeof(double) * (size_t)m * n;
+    const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, (160 * 1024) / std::max<size_t>(lds, 1)));
+    // (one wavefront per matrix at n <= 64 measured 2.4x slower than four)
+    if (lds <= 64 * 1024)
+        getrf_nopiv_kernel<true, 256><<<persistent_grid(count, per_cu), 256, lds, st>>>(m, n, A, lda, info, count);
+    else
+        getrf_nopiv_kernel<false, 256><<<persistent_grid(count, 2), 256, 0, st>>>(m, n, A, lda, info, count);
+    MAGMA_LAUNCH_CHECK();
+    return MAGMA_SUCCESS;
+}
+
+magma_int_t enqueue_getrs_nopiv(bool notrans, int n, int nrhs, double **A, int lda, double **B, int ldb, int count,
+                                hipStream_t st)
+{
+    if (n == 0 || nrhs == 0) return MAGMA_SUCCESS;
+    magma_int_t r;
+    if (notrans) {
+        r = launch_trsm<'L', 'L', 'N'>(n, nrhs, 1.0, A, lda, B, ldb, true, count, st);
+        if (r == MAGMA_SUCCESS) r = launch_trsm<'L', 'U', 'N'>(n, nrhs, 1.0, A, lda, B, ldb, false, count, st);
+    } else {
+        r = launch_trsm<'L', 'U', 'T'>(n, nrhs, 1.0, A, lda, B, ldb, false, count, st);
+        if (r == MAGMA_SUCCESS) r = launch_trsm<'L', 'L', 'T'>(n, nrhs, 1.0, A, lda, B, ldb, true, count, st);
+    }
+    return r;
+}
+
 }  // namespace
 }  // namespace magma_amd
 
@@ -793,25 +826,9 @@ magma_int_t magma_dgetrf_nopiv_batched(magma_int_t m, magma_int_t n, double **dA
         magma_xerbla(__func__, -arg);
         return arg;
     }
-    hipStream_t st = stream_of(queue);
     if (batchCount == 0) return MAGMA_SUCCESS;
-    if (m == 0 || n == 0) {
-        zero_info(info_array, batchCount, st);
-        return sync_unless_async(queue);
-    }
-    if (m > 2048 || n > 2048) return MAGMA_ERR_NOT_SUPPORTED;
-    const size_t lds = sizeof(double) * (size_t)m * n;
-    const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, (160 * 1024) / std::max<size_t>(lds, 1)));
-    // (one wavefront per matrix at n <= 64 measured 2.4x slower than four)
-    if (lds <= 64 * 1024) {
-        getrf_nopiv_kernel<true, 256><<<persistent_grid(batchCount, per_cu), 256, lds, st>>>(m, n, dA_array, ldda,
-                                                                                            info_array, batchCount);
-    } else {
-        getrf_nopiv_kernel<false, 256><<<persistent_grid(batchCount, 2), 256, 0, st>>>(m, n, dA_array, ldda,
-                                                                                      info_array, batchCount);
-    }
-    MAGMA_LAUNCH_CHECK();
-    return sync_unless_async(queue);
+    const magma_int_t r = enqueue_getrf_nopiv(m, n, dA_array, ldda, info_array, batchCount, stream_of(queue));
+    return r == MAGMA_SUCCESS ? sync_unless_async(queue) : r;
 }
 
 // include/magma_zbatched.h:593-598: the two triangular solves of getrs
@@ -834,19 +851,9 @@ magma_int_t magma_dgetrs_nopiv_batched(magma_trans_t trans, magma_int_t n, magma
     hipStream_t st = stream_of(queue);
     if (batchCount == 0) return MAGMA_SUCCESS;
     zero_info(info_array, batchCount, st);
-    if (n == 0 || nrhs == 0) return sync_unless_async(queue);
-    magma_int_t r;
-    if (trans == MagmaNoTrans) {
-        r = launch_trsm<'L', 'L', 'N'>(n, nrhs, 1.0, dA_array, ldda, dB_array, lddb, true, batchCount, st);
-        if (r == MAGMA_SUCCESS)
-            r = launch_trsm<'L', 'U', 'N'>(n, nrhs, 1.0, dA_array, ldda, dB_array, lddb, false, batchCount, st);
-    } else {
-        r = launch_trsm<'L', 'U', 'T'>(n, nrhs, 1.0, dA_array, ldda, dB_array, lddb, false, batchCount, st);
-        if (r == MAGMA_SUCCESS)
-            r = launch_trsm<'L', 'L', 'T'>(n, nrhs, 1.0, dA_array, ldda, dB_array, lddb, true, batchCount, st);
-    }
-    if (r != MAGMA_SUCCESS) return r;
-    return sync_unless_async(queue);
+    const magma_int_t r = enqueue_getrs_nopiv(trans == MagmaNoTrans, n, nrhs, dA_array, ldda, dB_array, lddb,
+                                              batchCount, st);
+    return r == MAGMA_SUCCESS ? sync_unless_async(queue) : r;
 }
 
 // include/magma_zbatched.h:601-606
@@ -864,15 +871,11 @@ magma_int_t magma_dgesv_nopiv_batched(magma_int_t n, magma_int_t nrhs, double **
         magma_xerbla(__func__, -arg);
         return arg;
     }
-    const bool was_async = queue && queue->async;
-    if (queue) queue->async = true;
-    magma_int_t r = magma_dgetrf_nopiv_batched(n, n, dA_array, ldda, info_array, batchCount, queue);
-    if (r == MAGMA_SUCCESS)
-        r = magma_dgetrs_nopiv_batched(MagmaNoTrans, n, nrhs, dA_array, ldda, dB_array, lddb, nullptr, batchCount,
-                                       queue);
-    if (queue) queue->async = was_async;
-    if (r != MAGMA_SUCCESS) return r;
-    return sync_unless_async(queue);
+    if (batchCount == 0) return MAGMA_SUCCESS;
+    hipStream_t st = stream_of(queue);
+    magma_int_t r = enqueue_getrf_nopiv(n, n, dA_array, ldda, info_array, batchCount, st);
+    if (r == MAGMA_SUCCESS) r = enqueue_getrs_nopiv(true, n, nrhs, dA_array, ldda, dB_array, lddb, batchCount, st);
+    return r == MAGMA_SUCCESS ? sync_unless_async(queue) : r;
 }
 
 // include/magma_zbatched.h:609-614; src/zgesv_rbt_batched.cpp: A <- U' A V,
@@ -913,13 +916,8 @@ magma_int_t magma_dgesv_rbt_batched(magma_int_t n, magma_int_t nrhs, double **dA
     MAGMA_LAUNCH_CHECK();
     rbt_vector_kernel<<<grid, 256, 0, st>>>(n, nrhs, dB_array, lddb, du, true, batchCount);
     MAGMA_LAUNCH_CHECK();
-    const bool was_async = queue && queue->async;
-    if (queue) queue->async = true;
-    magma_int_t r = magma_dgetrf_nopiv_batched(n, n, dA_array, ldda, dinfo_array, batchCount, queue);
-    if (r == MAGMA_SUCCESS)
-        r = magma_dgetrs_nopiv_batched(MagmaNoTrans, n, nrhs, dA_array, ldda, dB_array, lddb, nullptr, batchCount,
-                                       queue);
-    if (queue) queue->async = was_async;
+    magma_int_t r = enqueue_getrf_nopiv(n, n, dA_array, ldda, dinfo_array, batchCount, st);
+    if (r == MAGMA_SUCCESS) r = enqueue_getrs_nopiv(true, n, nrhs, dA_array, ldda, dB_array, lddb, batchCount, st);
     if (r != MAGMA_SUCCESS) return r;
     rbt_vector_kernel<<<grid, 256, 0, st>>>(n, nrhs, dB_array, lddb, dv, false, batchCount);
     MAGMA_LAUNCH_CHECK();

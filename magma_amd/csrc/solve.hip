@@ -30,6 +30,47 @@ __device__ void sweep(const double *T, int ldt, int n, double *X, int ldx, int n
 {
     const int tid = threadIdx.x;
     const bool forward = (lower != trans);
+    if (n <= kThreads) {
+        // one row per thread: the next step's entries of T (its column, or row
+        // when transposed, and its diagonal) are loaded while the current step
+        // runs, so each step does not wait on a global load
+        auto entry = [&](int k, int i) { return trans ? T[k + (size_t)i * ldt] : T[i + (size_t)k * ldt]; };
+        auto row_of = [&](int k) { return (forward ? k + 1 : 0) + tid; };
+        auto row_end = [&](int k) { return forward ? n : k; };
+        int k = forward ? 0 : n - 1;
+        double tc = 0.0, dc = 1.0;
+        if (row_of(k) < row_end(k)) tc = entry(k, row_of(k));
+        if (!unit && tid < nc) dc = T[k + (size_t)k * ldt];
+        for (int s = 0; s < n; ++s) {
+            k = forward ? s : n - 1 - s;
+            double tn = 0.0, dn = 1.0;
+            if (s + 1 < n) {
+                const int k1 = forward ? s + 1 : n - 2 - s;
+                if (row_of(k1) < row_end(k1)) tn = entry(k1, row_of(k1));
+                if (!unit && tid < nc) dn = T[k1 + (size_t)k1 * ldt];
+            }
+            if (!unit) {
+                if (tid < nc) {
+                    double *xk = X + (size_t)tid * ldx + k;
+                    // no-transpose sweeps skip a zero entry like reference dtrsm;
+                    // its transposed (dot-form) sweeps always divide
+                    if (trans || *xk != 0.0) *xk = *xk / dc;
+                }
+                __syncthreads();
+            }
+            const int i = row_of(k);
+            if (i < row_end(k)) {
+                for (int c = 0; c < nc; ++c) {
+                    const double xk = X[(size_t)c * ldx + k];
+                    X[(size_t)c * ldx + i] = fma(-xk, tc, X[(size_t)c * ldx + i]);
+                }
+            }
+            __syncthreads();
+            tc = tn;
+            dc = dn;
+        }
+        return;
+    }
     for (int s = 0; s < n; ++s) {
         const int k = forward ? s : n - 1 - s;
         if (!unit) {

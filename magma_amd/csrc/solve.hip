@@ -62,6 +62,7 @@ __device__ void sweep(const double *T, int ldt, int n, double *X, int ldx, int n
             if (i < row_end(k)) {
                 for (int c = 0; c < nc; ++c) {
                     const double xk = X[(size_t)c * ldx + k];
+                    if (!trans && xk == 0.0) continue;      // reference dtrsm's skip (x_k stays 0 after division)
                     X[(size_t)c * ldx + i] = fma(-xk, tc, X[(size_t)c * ldx + i]);
                 }
             }
@@ -91,6 +92,7 @@ __device__ void sweep(const double *T, int ldt, int n, double *X, int ldx, int n
             const double t = trans ? T[k + (size_t)i * ldt] : T[i + (size_t)k * ldt];
             for (int c = 0; c < nc; ++c) {
                 const double xk = X[(size_t)c * ldx + k];
+                if (!trans && xk == 0.0) continue;      // reference dtrsm's skip (x_k stays 0 after division)
                 X[(size_t)c * ldx + i] = fma(-xk, t, X[(size_t)c * ldx + i]);
             }
         }

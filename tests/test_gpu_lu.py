@@ -186,3 +186,32 @@ def test_argument_errors(queue):
     assert L.magma_dgesv_batched(4, 1, 0, 4, 0, 0, 2, 0, 1, q) == -6
     # quick returns touch nothing
     assert L.magma_dgetrf_batched(0, 0, 0, 1, 0, 0, 5, q) == 0
+
+
+@pytest.mark.parametrize("n", [5, 100, 300])
+def test_getrs_zero_skip_like_dtrsm(queue, oracle, n):
+    """Reference dtrsm skips the whole column update for a zero solution entry
+    (NoTrans), so an Inf below that pivot in L must not turn the solution into
+    NaN: with L(1,0) = 0 and b(1) = 0, x(1) is zero at its pivot and L(3,1) = Inf
+    is never used.  Bitwise equal to the oracle's dgetrs; covers the one-row-
+    per-thread sweep (n <= 256) and the general one."""
+    import magma_amd as ma
+    rng = np.random.default_rng(n)
+    count = 3
+    LU = rng.uniform(-1, 1, (count, n, n)) / n
+    LU[:, np.arange(n), np.arange(n)] = rng.uniform(1, 2, (count, n))
+    LU[:, 0, 1] = 0.0                      # L(1,0) = 0  (storage [j, i] = A(i, j))
+    LU[:, 1, 3] = np.inf                   # L(3,1) = Inf
+    B = rng.uniform(-1, 1, (count, 2, n))
+    B[:, :, 1] = 0.0
+    dA = ma.DeviceBatch(LU)
+    ipiv = ma.IntBatch(count, n)
+    ipiv.t[:] = ma._torch().arange(1, n + 1, dtype=ma._torch().int32, device="cuda")
+    dB = ma.DeviceBatch(B)
+    print(f"getrs zero skip n={n}", flush=True)
+    assert ma.dgetrs_batched(dA, ipiv, dB, queue) == 0
+    X = dB.numpy()
+    for s in range(count):
+        Xo = oracle.dgetrs(LU[s], np.arange(1, n + 1, dtype=np.int32), B[s])
+        assert np.all(np.isfinite(Xo))
+        assert same(X[s], Xo), s

@@ -62,6 +62,7 @@ typedef _Float16 half4 __attribute__((ext_vector_type(4)));
 typedef float float4v __attribute__((ext_vector_type(4)));
 
 constexpr int kT = 256;
+constexpr int kResidLoads = 32;     // loads of A in flight per thread in the FP64 residual
 constexpr int kWaves = kT / 64;
 constexpr int CH = 32;          // chunk width
 constexpr int kUPad = CH + 4;   // staging row (halves): 72-byte stride, conflict-free b64 reads
@@ -905,14 +906,15 @@ struct Mixed {
                 for (int g = 0; g < RG; ++g) acc[g] = (g < nr) ? B[o + (size_t)g * ldb] : 0.0;
                 double rs = 0.0;
                 const double *arow = A + o;
-                // eight independent loads in flight per step
+                // kResidLoads independent loads in flight per step
 #pragma unroll 1
-                for (int c0 = 0; c0 < n; c0 += 8) {
-                    double a8[8];
+                for (int c0 = 0; c0 < n; c0 += kResidLoads) {
+                    double a8[kResidLoads];
 #pragma unroll
-                    for (int t = 0; t < 8; ++t) a8[t] = (c0 + t < n) ? arow[(size_t)(c0 + t) * lda] : 0.0;
+                    for (int t = 0; t < kResidLoads; ++t)
+                        a8[t] = (c0 + t < n) ? arow[(size_t)(c0 + t) * lda] : 0.0;
 #pragma unroll
-                    for (int t = 0; t < 8; ++t) {
+                    for (int t = 0; t < kResidLoads; ++t) {
                         const int c = min(c0 + t, NMAX - 1);
                         rs += fabs(a8[t]);
 #pragma unroll

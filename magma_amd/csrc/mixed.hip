@@ -55,6 +55,7 @@ magma_int_t launch_potrf_small_v(const MatSet &s, int max_n, bool solve, bool va
 magma_int_t launch_getrf_mfma(const MatSet &s, int max_dim, hipStream_t st);
 magma_int_t launch_getrf_blocked(const MatSet &s, int max_m, hipStream_t st);
 magma_int_t launch_potrf_blocked(const MatSet &s, int max_n, hipStream_t st);
+magma_int_t zero_info(int *info, int count, hipStream_t st);
 
 namespace {
 
@@ -1261,7 +1262,12 @@ extern "C" magma_int_t magma_dsgesv_batched(magma_int_t n, magma_int_t nrhs,
         magma_xerbla(__func__, -info);
         return info;
     }
-    if (n == 0 || nrhs == 0 || batchCount == 0) return MAGMA_SUCCESS;
+    if (n == 0 || nrhs == 0 || batchCount == 0) {
+        // the quick return leaves iter = 0 and info = 0 for every matrix (src/zcgesv_gpu.cpp:174-193)
+        const hipStream_t st = stream_of(queue);
+        const magma_int_t e = zero_info(dinfo_array, batchCount, st);
+        return e != MAGMA_SUCCESS ? e : zero_info(diter_array, batchCount, st);
+    }
     return mixed_batched('L', n, nrhs, dA_array, ldda, dipiv_array, dB_array, lddb, dX_array, lddx, dinfo_array,
                          diter_array, batchCount, queue);
 }
@@ -1291,7 +1297,12 @@ extern "C" magma_int_t magma_dsposv_batched(magma_uplo_t uplo, magma_int_t n, ma
         magma_xerbla(__func__, -info);
         return info;
     }
-    if (n == 0 || nrhs == 0 || batchCount == 0) return MAGMA_SUCCESS;
+    if (n == 0 || nrhs == 0 || batchCount == 0) {
+        // the quick return leaves iter = 0 and info = 0 for every matrix (src/zcposv_gpu.cpp:163-182)
+        const hipStream_t st = stream_of(queue);
+        const magma_int_t e = zero_info(dinfo_array, batchCount, st);
+        return e != MAGMA_SUCCESS ? e : zero_info(diter_array, batchCount, st);
+    }
     return mixed_batched('C', n, nrhs, dA_array, ldda, nullptr, dB_array, lddb, dX_array, lddx, dinfo_array,
                          diter_array, batchCount, queue);
 }

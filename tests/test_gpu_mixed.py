@@ -188,6 +188,28 @@ def test_dsgesv_argument_errors(queue):
     assert L.magma_dsposv_batched(ma.MagmaLower, 4, 1, 0, 4, 0, 3, 0, 4, 0, 0, 1, q) == -7
 
 
+@pytest.mark.parametrize("n,nrhs", [(0, 1), (4, 0), (0, 0)])
+def test_quick_return_sets_iter_and_info(queue, n, nrhs):
+    """n = 0 or nrhs = 0: nothing is factored or solved, and every matrix
+    gets iter = 0, info = 0, as magma_dsgesv_gpu / magma_dsposv_gpu leave
+    them (src/zcgesv_gpu.cpp:174-193, src/zcposv_gpu.cpp:163-182)."""
+    import magma_amd as ma
+    L = ma.load()
+    count = 5
+    for kind in ("L", "C"):
+        info = ma.dev_ints(np.full(count, 77))
+        it = ma.dev_ints(np.full(count, 99))
+        print(f"quick return kind={kind} n={n} nrhs={nrhs}", flush=True)
+        if kind == "L":
+            ret = L.magma_dsgesv_batched(n, nrhs, 0, 4, 0, 0, 4, 0, 4, info.data_ptr(), it.data_ptr(), count, queue.handle)
+        else:
+            ret = L.magma_dsposv_batched(ma.MagmaLower, n, nrhs, 0, 4, 0, 4, 0, 4, info.data_ptr(), it.data_ptr(),
+                                         count, queue.handle)
+        assert ret == 0
+        queue.sync()
+        assert np.all(info.cpu().numpy()[:count] == 0) and np.all(it.cpu().numpy()[:count] == 0), kind
+
+
 # ---------------------------------------------------------------- vbatched ---
 def run_mixed_v(queue, mats, rhs, kind="L"):
     """magma_dsgesv_vbatched / magma_dsposv_vbatched on matrices of different

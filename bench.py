@@ -220,8 +220,30 @@ def workload_chol64(args, torch, ma, L, q):
                                                                      gen.rhs_batch(1, k, n, nrhs))))
 
 
+def gpu_backward_errors(torch, A, X, B, chunk=512):
+    """Per-matrix FP64 backward error ||b - A x||_inf / (||A||_inf ||x||_inf)
+    divided by n eps64, computed on the device for every matrix and every
+    right-hand side (A: (count, n, n) column-major storage, [s, j, i] = A(i, j);
+    X, B: (count, nrhs, n)).  Returns a float64 numpy array (count,)."""
+    count, n, _ = A.shape
+    out = []
+    for c0 in range(0, count, chunk):
+        a = A[c0:c0 + chunk]
+        x = X[c0:c0 + chunk]
+        b = B[c0:c0 + chunk]
+        # (x' A')' = A x: row vectors times the column-major storage
+        ax = torch.bmm(x.contiguous(), a.contiguous())
+        r = (b - ax).abs().amax(dim=2)
+        anrm = a.abs().sum(dim=1).amax(dim=1, keepdim=True)        # max row sum
+        xn = x.abs().amax(dim=2)
+        be = r / (anrm * xn) / (n * EPS)
+        out.append(be.amax(dim=1))
+    return torch.cat(out).cpu().numpy()
+
+
 def workload_cond256(args, torch, ma, L, q):
-    """cfg4: plain FP64 getrf+getrs vs mixed dsgesv on cond-controlled n=256."""
+    """8192 matrices of order 256 with cond log-uniform in [1e2, 1e6], one
+    right-hand side: plain FP64 getrf + getrs against mixed dsgesv."""
     from magma_amd import gen
     count, n, nrhs = args.batch or 8192, 256, 1
     log(f"generating {count} conditioned matrices n={n}")
@@ -231,8 +253,8 @@ def workload_cond256(args, torch, ma, L, q):
     B0 = gen.uniform_pm1_torch(103, 0, count, n * nrhs, "cuda").view(count, nrhs, n).contiguous()
     ipiv = ma.IntBatch(count, n)
     info = torch.zeros(count, dtype=torch.int32, device="cuda")
-    have_mixed = L.exported("magma_dsgesv_batched")
     fact = Timer(torch)
+    step_t = Timer(torch)
 
     def make(i):
         A = A0.clone(); B = B0.clone()
@@ -240,21 +262,41 @@ def workload_cond256(args, torch, ma, L, q):
 
     def step_plain(c, timer):
         A, B, pa, pb = c
+        e0 = step_t.start() if timer else None
         e = fact.start() if timer else None
         L.magma_dgetrf_batched(n, n, pa.data_ptr(), n, ipiv.ptr, info.data_ptr(), count, q.handle)
         if timer:
             fact.stop(e)
         L.magma_dgetrs_batched(ma.MagmaNoTrans, n, nrhs, pa.data_ptr(), n, ipiv.ptr, pb.data_ptr(), n, count,
                                q.handle)
+        if timer:
+            step_t.stop(e0)
 
     res = {}
     wall, _, copies = run_fixed(args, torch, ma, L, q, make, step_plain, count, count, "matrices")
     plain_ms = wall / args.steps * 1e3
     fact_ms = float(np.mean(fact.ms()))
-    # backward error of the plain solution on a sample
-    A_s = A0[:64].cpu().numpy(); B_s = B0[:64].cpu().numpy(); X_s = copies[-1][1][:64].cpu().numpy()
-    del copies
-    res["plain"] = dict(solves_per_s=count / (plain_ms * 1e-3), ms_per_step=plain_ms, factor_kernel_ms=fact_ms)
+    plain_steps = np.asarray(step_t.ms())
+    res["plain"] = dict(solves_per_s=count / (plain_ms * 1e-3), ms_per_step=plain_ms, factor_kernel_ms=fact_ms,
+                        step_p50_ms=float(np.percentile(plain_steps, 50)),
+                        step_p99_ms=float(np.percentile(plain_steps, 99)))
+    check = {}
+    # every plain solution: FP64 backward error on the device
+    Xp = copies[-1][1]
+    be_p = gpu_backward_errors(torch, A0, Xp, B0)
+    check["plain_backward_err_over_n_eps_max"] = float(be_p.max())
+    check["plain_backward_err_above_n_eps"] = int(np.sum(~(be_p <= 1.0)))
+    check["plain_info_nonzero"] = int((info != 0).sum())
+    if args.check:
+        # pivots, info and factors of a sample against the oracle, bit for bit
+        from oracle import pyoracle
+        k = 32
+        LUo, ipo, info_o = pyoracle.batch_dgetrf(A0[:k].cpu().numpy())
+        check["plain_oracle_sample"] = k
+        check["plain_ipiv_equal"] = bool(np.array_equal(ipiv.numpy()[:k], ipo))
+        check["plain_info_equal"] = bool(np.array_equal(info[:k].cpu().numpy(), info_o))
+        check["plain_lu_bitexact"] = bool(np.array_equal(copies[-1][0][:k].cpu().numpy(), LUo))
+    del copies, Xp
     # the plain factorization against both of its bounds
     fbytes = count * getrf_bytes(n)
     res["plain"]["roofline"] = dict(
@@ -264,64 +306,72 @@ def workload_cond256(args, torch, ma, L, q):
         **measured_traffic("cond256_mixed_vs_plain", "getrf_mfma_kernel", fbytes))
     if "traffic" in res["plain"]["roofline"]:
         res["plain"]["roofline"]["traffic_GBps"] = res["plain"]["roofline"]["traffic"] / (fact_ms * 1e-3) / 1e9
-    flops = count * getrf_flops(n)
-    roof = {"bound": "mfma", "achieved": flops / (fact_ms * 1e-3) / 1e12, "peak": FP64_PEAK / 1e12,
-            "unit": "TFLOP/s", "kernel": "getrf_mfma_kernel<256> (FP64 MFMA trailing update; FP64 peak)",
-            "kernel_ms": fact_ms, "flops_per_launch": flops}
-    value, ms = res["plain"]["solves_per_s"], plain_ms
-    check = {}
+
+    X = torch.zeros_like(B0)
+    px = ma.ptr_array(X, count, n * nrhs)
+    it = torch.zeros(count, dtype=torch.int32, device="cuda")
+    mix = Timer(torch)
+
+    def make_m(i):
+        # A is read-only unless a matrix falls back to FP64 (then it holds
+        # the LU factors), so every step gets its own copy as well
+        A = A0.clone()
+        return (A, B0, ma.ptr_array(A, count, n * n), ma.ptr_array(B0, count, n * nrhs))
+
+    def step_mixed(c, timer):
+        A, B, pa, pb = c
+        e = mix.start() if timer else None
+        L.magma_dsgesv_batched(n, nrhs, pa.data_ptr(), n, ipiv.ptr, pb.data_ptr(), n, px.data_ptr(), n,
+                               info.data_ptr(), it.data_ptr(), count, q.handle)
+        if timer:
+            mix.stop(e)
+
+    wall_m, _, _ = run_fixed(args, torch, ma, L, q, make_m, step_mixed, count, count, "matrices")
+    mixed_ms = wall_m / args.steps * 1e3
+    mixed_steps = np.asarray(mix.ms())
+    iters = it.cpu().numpy()
+    res["mixed"] = dict(solves_per_s=count / (mixed_ms * 1e-3), ms_per_step=mixed_ms,
+                        kernel_ms=float(np.mean(mixed_steps)),
+                        step_p50_ms=float(np.percentile(mixed_steps, 50)),
+                        step_p99_ms=float(np.percentile(mixed_steps, 99)),
+                        sweeps_mean=float(np.mean(np.maximum(iters, 0))),
+                        sweeps_max=int(np.max(iters)),
+                        fallbacks=int(np.sum(iters < 0)))
+    res["mixed_over_plain"] = res["mixed"]["solves_per_s"] / res["plain"]["solves_per_s"]
+    be_m = gpu_backward_errors(torch, A0, X, B0)
+    check["mixed_backward_err_over_n_eps_max"] = float(be_m.max())
+    check["mixed_backward_err_above_n_eps"] = int(np.sum(~(be_m <= 1.0)))
+    check["backward_err_checked"] = f"all {count} solutions, FP64 residual on the device"
     if args.check:
         from oracle import pyoracle
-        errs = [pyoracle.solve_error(A_s[s], X_s[s], B_s[s]) for s in range(0, 64, 8)]
-        check["plain_max_backward_err_over_n_eps"] = float(max(errs))
-    if have_mixed:
-        X = torch.zeros_like(B0)
-        px = ma.ptr_array(X, count, n * nrhs)
-        it = torch.zeros(count, dtype=torch.int32, device="cuda")
-        mix = Timer(torch)
-
-        def make_m(i):
-            # A is read-only unless a matrix falls back to FP64 (then it holds
-            # the LU factors), so every step gets its own copy as well
-            A = A0.clone()
-            return (A, B0, ma.ptr_array(A, count, n * n), ma.ptr_array(B0, count, n * nrhs))
-
-        def step_mixed(c, timer):
-            A, B, pa, pb = c
-            e = mix.start() if timer else None
-            L.magma_dsgesv_batched(n, nrhs, pa.data_ptr(), n, ipiv.ptr, pb.data_ptr(), n, px.data_ptr(), n,
-                                   info.data_ptr(), it.data_ptr(), count, q.handle)
-            if timer:
-                mix.stop(e)
-
-        wall_m, _, _ = run_fixed(args, torch, ma, L, q, make_m, step_mixed, count, count, "matrices")
-        mixed_ms = wall_m / args.steps * 1e3
-        iters = it.cpu().numpy()
-        res["mixed"] = dict(solves_per_s=count / (mixed_ms * 1e-3), ms_per_step=mixed_ms,
-                            kernel_ms=float(np.mean(mix.ms())),
-                            sweeps_mean=float(np.mean(np.maximum(iters, 0))),
-                            fallbacks=int(np.sum(iters < 0)))
-        res["mixed_over_plain"] = res["mixed"]["solves_per_s"] / res["plain"]["solves_per_s"]
-        value, ms = res["mixed"]["solves_per_s"], mixed_ms
-        if args.check:
-            from oracle import pyoracle
-            Xm = X[:64].cpu().numpy()
-            errs = [pyoracle.solve_error(A_s[s], Xm[s], B_s[s]) / n for s in range(64)]
-            check["mixed_max_backward_err_over_n_eps"] = float(max(errs) * n)
-            check["mixed_max_backward_err_ratio_to_n_eps"] = float(max(errs))
-        byt = count * (8.0 * n * n + 16.0 * n * nrhs + 8)
-        roof = {"bound": "hbm", "achieved": byt / (res["mixed"]["kernel_ms"] * 1e-3) / 1e9,
-                "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "kernel": "mixed_kernel (FP32 factor + FP64 refinement; the dsgesv call's dominant launch)",
-                "kernel_ms": res["mixed"]["kernel_ms"], "bytes_per_launch": byt}
-        roof.update(measured_traffic("cond256_mixed_vs_plain", "mixed_kernel", byt))
-    return dict(value=value, unit="solves/s", ms=ms, dtype="f64",
+        k = 32
+        A_s = A0[:k].cpu().numpy(); B_s = B0[:k].cpu().numpy(); Xm = X[:k].cpu().numpy()
+        check["mixed_oracle_residual_sample_max_over_n_eps"] = float(
+            max(pyoracle.solve_error(A_s[s], Xm[s], B_s[s]) for s in range(k)))
+    byt = count * (8.0 * n * n + 16.0 * n * nrhs + 8)
+    roof = {"bound": "hbm", "achieved": byt / (res["mixed"]["kernel_ms"] * 1e-3) / 1e9,
+            "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "kernel": "mixed_kernel (FP32 factor + FP64 refinement; the dsgesv call's dominant launch)",
+            "kernel_ms": res["mixed"]["kernel_ms"], "bytes_per_launch": byt}
+    roof.update(measured_traffic("cond256_mixed_vs_plain", "mixed_kernel", byt))
+    serve = None
+    if args.serve_leg > 0:
+        sargs = argparse.Namespace(**vars(args))
+        sargs.serve_seconds = args.serve_leg
+        sr = workload_serve(sargs, torch, ma, L, q)
+        c = sr["config"]
+        serve = {k: c[k] for k in ("arrival_rate_per_s", "seconds", "requests", "latency_p50_ms", "latency_p99_ms",
+                                   "latency_max_ms", "fallbacks_fp64", "backward_err_over_n_eps_max",
+                                   "reused_factorizations")}
+        serve["solves_per_s"] = sr["value"]
+        serve["workload"] = "serve_poisson (Poisson arrivals, n log-uniform [16,512], nrhs 1-16, 50% SPD, " \
+                            "20% repeat right-hand sides, 64 GB budget)"
+    return dict(value=res["mixed"]["solves_per_s"], unit="solves/s", ms=mixed_ms, dtype="f64",
                 config={"workload": "cond256_mixed_vs_plain", "batch": count, "n": n, "nrhs": nrhs,
                         "inputs": "A = U diag(s) V', U,V from QR of N(0,1) (seed 3), cond log-uniform [1e2,1e6]",
-                        "modes": res},
-                roofline=roof, check=check, step_ms=None,
-                cpu=dict(kind="dgesv", n=n, nrhs=nrhs, gen=lambda k: (A0[:k].cpu().numpy(),
-                                                                     B0[:k].cpu().numpy())))
+                        "modes": res, "serve": serve},
+                roofline=roof, check=check, step_ms=list(mixed_steps),
+                cpu=dict(kind="dgesv", n=n, nrhs=nrhs, gen=lambda k: (A0[:k].cpu().numpy(), B0[:k].cpu().numpy())))
 
 
 def workload_vbatch(args, torch, ma, L, q):
@@ -670,8 +720,13 @@ WORKLOADS = {
 
 
 def cpu_baseline(spec, seconds):
-    """Times the oracle (C, OpenMP over the batch) and, if present, a
-    third-party LAPACK on a bounded sample of the same workload."""
+    """The reference's CPU path on a bounded sample of the same workload:
+    LAPACK dgetrf + dgetrs (dpotrf + dpotrs) called once per matrix in an
+    OpenMP loop over the batch with LAPACK single-threaded, as MAGMA's testers
+    run it (testing/testing_zgesv_batched.cpp:159-186).  The LAPACK is the
+    host's OpenBLAS when one is found (then it is `value`); the oracle's
+    netlib-order C restatement, under the same OpenMP loop, is reported beside
+    it (and is `value` when no LAPACK is found)."""
     if spec is None:
         return None
     from oracle import pyoracle
@@ -693,20 +748,25 @@ def cpu_baseline(spec, seconds):
     for _ in range(reps):
         fn(A, B, nthreads=threads)
     dt = (dt + time.perf_counter() - t0) / (reps + 1)
-    out = {"value": k2 / dt, "unit": "solves/s", "cores": threads, "kind": "port",
-           "sample": f"{k2} matrices of the workload (n={n}, nrhs={nrhs}) x {reps + 1} passes, oracle {kind} "
-                     f"(netlib-order C restatement) under OpenMP, {threads} threads, "
-                     f"{dt * (reps + 1):.1f} s"}
+    port = {"value": k2 / dt, "unit": "solves/s", "cores": threads, "kind": "port",
+            "sample": f"{k2} matrices of the workload (n={n}, nrhs={nrhs}) x {reps + 1} passes, oracle {kind} "
+                      f"(netlib-order C restatement) under OpenMP, {threads} threads, {dt * (reps + 1):.1f} s"}
     cfg = pyoracle.lapack()
-    if cfg:
-        lreps = max(1, reps + 1)
-        t0 = time.perf_counter()
-        for _ in range(lreps):
-            lfn(A, B, nthreads=threads)
-        dl = (time.perf_counter() - t0) / lreps
-        out["lapack"] = {"value": k2 / dl, "unit": "solves/s", "library": cfg.split()[0] if cfg else "",
-                         "config": cfg, "cores": threads}
-    return out
+    if not cfg:
+        return port
+    lreps = max(2, reps + 1)
+    t0 = time.perf_counter()
+    for _ in range(lreps):
+        lfn(A, B, nthreads=threads)
+    dl = (time.perf_counter() - t0) / lreps
+    routine = "dgetrf + dgetrs" if kind == "dgesv" else "dpotrf + dpotrs"
+    return {"value": k2 / dl, "unit": "solves/s", "cores": threads, "kind": "reference",
+            "path": f"the reference's CPU path: LAPACK {routine} per matrix, OpenMP loop over the batch "
+                    f"(testing/testing_zgesv_batched.cpp:159-186), LAPACK = {cfg.split()[0]} single-threaded",
+            "library": cfg,
+            "sample": f"{k2} matrices of the workload (n={n}, nrhs={nrhs}) x {lreps} passes, {threads} threads, "
+                      f"{dl * lreps:.1f} s",
+            "port": port}
 
 
 def main():
@@ -718,6 +778,9 @@ def main():
     ap.add_argument("--batch", type=int, default=0, help="override the batch count (cond256 only)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--serve-seconds", type=float, default=30.0, help="trace length of serve_poisson")
+    ap.add_argument("--serve-leg", type=float, default=5.0,
+                    help="seconds of the serving trace replayed after the cond256 workload (0: none); its "
+                         "p50/p99 latency go into config.serve")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one 1 GiB device-to-device copy (the reference kernel for scaling "
                          "FETCH_SIZE / WRITE_SIZE counters to bytes)")

@@ -42,8 +42,8 @@ magma_int_t magma_dsgesv_batched(magma_int_t n, magma_int_t nrhs,
  * a_ii 2^-2e_i in [0.5, 2), split-FP16 MFMA updates, FP64 refinement; batched
  * analogue of magma_dsposv_gpu (src/zcposv_gpu.cpp, include/magma_zc.h:68-77)
  * with the same argument meaning and the same iter codes as
- * magma_dsgesv_batched.  MagmaUpper is rejected with -1.  On fallback A holds
- * the FP64 Cholesky factor and info[i] > 0 reports a matrix that is not
+ * magma_dsgesv_batched.  MagmaUpper reads the upper triangle (A = U'U).  On
+ * fallback A holds the FP64 Cholesky factor in the triangle given by uplo and info[i] > 0 reports a matrix that is not
  * positive definite. */
 magma_int_t magma_dsposv_batched(magma_uplo_t uplo, magma_int_t n, magma_int_t nrhs,
                                  double **dA_array, magma_int_t ldda,

@@ -38,9 +38,10 @@ magma_int_t magma_dgesv_batched(magma_int_t n, magma_int_t nrhs, double **dA_arr
                                 magma_int_t **dipiv_array, double **dB_array, magma_int_t lddb,
                                 magma_int_t *dinfo_array, magma_int_t batchCount, magma_queue_t queue);
 
-/* include/magma_zbatched.h:324-328 -- Cholesky A = L*L' (MagmaLower; MagmaUpper
- * is rejected with -1 exactly as the reference's batched kernels do,
- * magmablas/zpotf2_kernels.cu:153-156) */
+/* include/magma_zbatched.h:324-328 -- Cholesky A = L*L' (MagmaLower) or
+ * A = U'*U (MagmaUpper, LAPACK's upper dpotrf; the reference's batched kernels
+ * reject Upper with -1, magmablas/zpotf2_kernels.cu:153-156).  Upper reads and
+ * writes only the upper triangle. */
 magma_int_t magma_dpotrf_batched(magma_uplo_t uplo, magma_int_t n, double **dA_array, magma_int_t lda,
                                  magma_int_t *info_array, magma_int_t batchCount, magma_queue_t queue);
 

@@ -617,12 +617,6 @@ __global__ void __launch_bounds__(256) perm_identity_kernel(int n, double **A_ar
     }
 }
 
-magma_int_t sync_unless_async(magma_queue_t queue)
-{
-    if (!(queue && queue->async)) MAGMA_HIP_CHECK(hipStreamSynchronize(stream_of(queue)));
-    return MAGMA_SUCCESS;
-}
-
 template <bool T, bool U>
 magma_int_t launch_trsv(int n, double **A, int lda, double **x, int incx, bool unit, int count, hipStream_t st)
 {

@@ -154,7 +154,9 @@ magma_int_t magma_dgesv_batched(magma_int_t n, magma_int_t nrhs, double **dA_arr
     return finish(launch_getrf(s, n, n, true, st), queue);
 }
 
-// src/zpotrf_batched.cpp:290-341; Upper rejected as magmablas/zpotf2_kernels.cu:153-156 does
+// src/zpotrf_batched.cpp:290-341.  The reference rejects Upper
+// (src/zpotrf_batched.cpp:106-109); here it is LAPACK's upper dpotrf, through
+// the lower kernels between two in-place triangle swaps (uplo.hip).
 magma_int_t magma_dpotrf_batched(magma_uplo_t uplo, magma_int_t n, double **dA_array, magma_int_t lda,
                                  magma_int_t *info_array, magma_int_t batchCount, magma_queue_t queue)
 {
@@ -163,10 +165,6 @@ magma_int_t magma_dpotrf_batched(magma_uplo_t uplo, magma_int_t n, double **dA_a
     if (uplo != MagmaUpper && uplo != MagmaLower) arginfo = -1;
     else if (n < 0) arginfo = -2;
     else if (lda < std::max(1, n)) arginfo = -4;
-    else if (uplo == MagmaUpper) {
-        std::fprintf(stderr, "%s: uplo=upper is not yet implemented\n", __func__);
-        arginfo = -1;
-    }
     if (arginfo != 0) {
         zero_info(info_array, batchCount, st);
         magma_xerbla(__func__, -arginfo);
@@ -180,6 +178,12 @@ magma_int_t magma_dpotrf_batched(magma_uplo_t uplo, magma_int_t n, double **dA_a
     s.n = n; s.m = n; s.lda = lda;
     s.A = dA_array; s.info = info_array;
     s.count = batchCount;
+    if (uplo == MagmaUpper) {
+        magma_int_t r = launch_swap_triangles(s, st);
+        if (r == MAGMA_SUCCESS) r = launch_potrf(s, n, false, st);
+        if (r == MAGMA_SUCCESS) r = launch_swap_triangles(s, st);
+        return finish(r, queue);
+    }
     return finish(launch_potrf(s, n, false, st), queue);
 }
 
@@ -218,10 +222,6 @@ magma_int_t magma_dposv_batched(magma_uplo_t uplo, magma_int_t n, magma_int_t nr
     else if (nrhs < 0) info = -3;
     else if (ldda < std::max(1, n)) info = -5;
     else if (lddb < std::max(1, n)) info = -7;
-    else if (uplo == MagmaUpper) {
-        std::fprintf(stderr, "%s: uplo=upper is not yet implemented\n", __func__);
-        info = -1;
-    }
     if (info != 0) {
         magma_xerbla(__func__, -info);
         return info;
@@ -236,6 +236,15 @@ magma_int_t magma_dposv_batched(magma_uplo_t uplo, magma_int_t n, magma_int_t nr
     s.A = dA_array; s.info = dinfo_array;
     s.B = dB_array; s.nrhs = nrhs; s.ldb = lddb;
     s.count = batchCount;
+    if (uplo == MagmaUpper) {
+        // U from the swapped lower factorization, then dpotrs('U'): U' solve,
+        // U solve; a matrix with info > 0 keeps its right-hand sides
+        magma_int_t r = launch_swap_triangles(s, st);
+        if (r == MAGMA_SUCCESS) r = launch_potrf(s, n, false, st);
+        if (r == MAGMA_SUCCESS) r = launch_swap_triangles(s, st);
+        if (r == MAGMA_SUCCESS) r = launch_potrs(s, 'U', n, nrhs, st);
+        return finish(r, queue);
+    }
     return finish(launch_potrf(s, n, true, st), queue);
 }
 

@@ -54,3 +54,16 @@ inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
         hipError_t e_ = hipGetLastError();                                                  \
         if (e_ != hipSuccess) return magma_amd::report_hip_error(e_, "kernel launch", __FILE__, __LINE__); \
     } while (0)
+
+namespace magma_amd {
+
+// The reference drivers end with magma_queue_sync (src/zgetrf_batched.cpp:354):
+// results are ready when the call returns, unless the queue was made
+// asynchronous for graph capture.
+inline magma_int_t sync_unless_async(magma_queue_t queue)
+{
+    if (!(queue && queue->async)) MAGMA_HIP_CHECK(hipStreamSynchronize(stream_of(queue)));
+    return MAGMA_SUCCESS;
+}
+
+}  // namespace magma_amd

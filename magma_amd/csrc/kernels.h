@@ -50,6 +50,11 @@ magma_int_t launch_potrf(const MatSet &s, int max_n, bool fused_solve, hipStream
 magma_int_t launch_getrs(const MatSet &s, char trans, int max_n, int max_nrhs, hipStream_t st);
 magma_int_t launch_potrs(const MatSet &s, char uplo, int max_n, int max_nrhs, hipStream_t st);
 
+// Swaps A(i, j) and A(j, i) for every i > j in place (0 < n <= lda; other
+// matrices are skipped).  Two swaps around the lower-triangular Cholesky
+// kernels factor an upper-stored matrix with LAPACK's upper dpotrf arithmetic.
+magma_int_t launch_swap_triangles(const MatSet &s, hipStream_t st);
+
 // GEMM C = alpha op(A) op(B) + beta C for a set of (possibly different) sizes.
 struct GemmSet {
     char ta = 'N', tb = 'N';

@@ -42,9 +42,16 @@ struct MixedSet {
     int count = 0;
 };
 
-// Grid of persistent workgroups used for max order max_n (sizes Wslot).
-int mixed_grid(int count, int max_n);
-magma_int_t launch_mixed(const MixedSet &g, int max_n, hipStream_t st);
+// Kernel variant and grid of persistent workgroups for `count` matrices of
+// order at most max_n.  The variant is read from the tuning table once, so the
+// grid that sizes the factor slots (Wslot, one per workgroup) is the grid the
+// launch uses even if the table changes in between.
+struct MixedPlan {
+    int variant = 0;
+    int grid = 0;
+};
+MixedPlan mixed_plan(int count, int max_n);
+magma_int_t launch_mixed(const MixedSet &g, int max_n, const MixedPlan &plan, hipStream_t st);
 
 // FP64 fallback for the matrices the mixed kernel listed: X <- B, then the
 // plain factorization and solve in place (A receives the FP64 factors).

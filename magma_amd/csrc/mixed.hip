@@ -227,12 +227,14 @@ struct Mixed {
             double vv[LC][RPL];
 #pragma unroll
             for (int u = 0; u < LC; ++u) {
-                const double *col = A + (size_t)(c0 + min(cg + kWaves * u, nb - 1)) * lda;
+                const int cl = c0 + min(cg + kWaves * u, nb - 1);
+                const double *col = A + (size_t)cl * lda;
 #pragma unroll
                 for (int r = 0; r < RPL; ++r) {
                     const int o = lane + 64 * r;
-                    // Cholesky reads the lower part only (upper entries are never used)
-                    const bool in = o < n && (!chol || o >= c0);
+                    // Cholesky reads the lower triangle only (the strict upper
+                    // triangle may hold anything, as in LAPACK)
+                    const bool in = o < n && (!chol || o >= cl);
                     vv[u][r] = in ? col[min(o, n - 1)] : 0.0;
                 }
             }
@@ -691,7 +693,10 @@ struct Mixed {
                     frexp(d, &e);
                     e >>= 1;
                 } else {
-                    sm.flag = (d != d || !__builtin_isfinite(d)) ? 1 : 2;
+                    // bit 0: not finite, bit 1: not positive; a matrix with
+                    // both kinds reports the first (iter -2) whichever
+                    // thread stores last
+                    atomicOr(&sm.flag, (d != d || !__builtin_isfinite(d)) ? 1 : 2);
                 }
                 sm.cexp[o] = e;
             }
@@ -699,7 +704,7 @@ struct Mixed {
         __syncthreads();
         const int f0 = sm.flag;
         __syncthreads();
-        if (f0) return f0 == 1 ? -2 : -3;
+        if (f0) return (f0 & 1) ? -2 : -3;
         int status = 0;
 #pragma unroll 1
         for (int c0 = 0; c0 < n; c0 += NB) {
@@ -907,13 +912,18 @@ struct Mixed {
                 for (int g = 0; g < RG; ++g) acc[g] = (g < nr) ? B[o + (size_t)g * ldb] : 0.0;
                 double rs = 0.0;
                 const double *arow = A + o;
+                // Cholesky reads the lower triangle only, as dsymv does in
+                // dsposv's residual: A(o, c) for c > o is A(c, o)
+                const double *acol = A + (size_t)o * lda;
                 // kResidLoads independent loads in flight per step
 #pragma unroll 1
                 for (int c0 = 0; c0 < n; c0 += kResidLoads) {
                     double a8[kResidLoads];
 #pragma unroll
-                    for (int t = 0; t < kResidLoads; ++t)
-                        a8[t] = (c0 + t < n) ? arow[(size_t)(c0 + t) * lda] : 0.0;
+                    for (int t = 0; t < kResidLoads; ++t) {
+                        const int c = c0 + t;
+                        a8[t] = (c < n) ? ((chol && c > o) ? acol[c] : arow[(size_t)c * lda]) : 0.0;
+                    }
 #pragma unroll
                     for (int t = 0; t < kResidLoads; ++t) {
                         const int c = min(c0 + t, NMAX - 1);
@@ -1096,32 +1106,39 @@ static int resident_per_cu(K kernel)
 //             memory-side cache between their passes), 4 (as 2 with registers
 //             held to three workgroups per CU)
 //   n <= 512: 0 (NB 64, RG 8), 1 (NB 32, RG 8)
+// The variant is chosen per size class of max_n (classes 4, 5, 6: orders up
+// to 128, 256, 512); the two smaller classes share the NMAX = 256 kernels.
 static int mixed_variant(int max_n)
 {
-    return tuned_variant(kTuneMixed, size_class_of(max_n <= 256 ? 256 : 512));
+    return tuned_variant(kTuneMixed, size_class_of(max_n <= 128 ? 128 : (max_n <= 256 ? 256 : 512)));
 }
 
-int mixed_grid(int count, int max_n)
+// Workgroups resident per CU of variant v for orders up to max_n.
+static int mixed_per_cu(int v, int max_n)
 {
-    const int v = mixed_variant(max_n);
-    int per_cu;
     if (max_n <= 256) {
-        per_cu = v == 0 ? resident_per_cu(mixed_kernel<256, 128, 16, 1>)
-               : v == 1 ? resident_per_cu(mixed_kernel<256, 64, 8, 1>)
-               : v == 2 ? resident_per_cu(mixed_kernel<256, 32, 8, 2>)
-               : v == 3 ? 1 : resident_per_cu(mixed_kernel<256, 32, 8, 3>);
-    } else {
-        per_cu = v == 0 ? resident_per_cu(mixed_kernel<512, 64, 8, 1>) : resident_per_cu(mixed_kernel<512, 32, 8, 1>);
+        return v == 0 ? resident_per_cu(mixed_kernel<256, 128, 16, 1>)
+             : v == 1 ? resident_per_cu(mixed_kernel<256, 64, 8, 1>)
+             : v == 2 ? resident_per_cu(mixed_kernel<256, 32, 8, 2>)
+             : v == 3 ? 1 : resident_per_cu(mixed_kernel<256, 32, 8, 3>);
     }
-    return persistent_grid(count, per_cu);
+    return v == 0 ? resident_per_cu(mixed_kernel<512, 64, 8, 1>) : resident_per_cu(mixed_kernel<512, 32, 8, 1>);
 }
 
-magma_int_t launch_mixed(const MixedSet &g, int max_n, hipStream_t st)
+MixedPlan mixed_plan(int count, int max_n)
+{
+    MixedPlan p;
+    p.variant = mixed_variant(max_n);
+    p.grid = persistent_grid(count, mixed_per_cu(p.variant, max_n));
+    return p;
+}
+
+magma_int_t launch_mixed(const MixedSet &g, int max_n, const MixedPlan &plan, hipStream_t st)
 {
     if (g.count <= 0) return MAGMA_SUCCESS;
     if (max_n > kMixedMaxN) return MAGMA_ERR_NOT_SUPPORTED;
-    const int grid = mixed_grid(g.count, max_n);
-    const int v = mixed_variant(max_n);
+    const int grid = plan.grid, v = plan.variant;
+    if (grid <= 0) return MAGMA_ERR_ILLEGAL_VALUE;
     if (max_n <= 256) {
         if (v == 0) mixed_kernel<256, 128, 16, 1><<<grid, kT, 0, st>>>(g);
         else if (v == 1) mixed_kernel<256, 64, 8, 1><<<grid, kT, 0, st>>>(g);
@@ -1195,16 +1212,16 @@ using namespace magma_amd;
 namespace {
 
 // shared driver of the fixed-size entry points
-magma_int_t mixed_batched(char kind, magma_int_t n, magma_int_t nrhs, double **dA_array, magma_int_t ldda,
+magma_int_t mixed_batched(char kind, bool upper, magma_int_t n, magma_int_t nrhs, double **dA_array, magma_int_t ldda,
                           magma_int_t **dipiv_array, double **dB_array, magma_int_t lddb, double **dX_array,
                           magma_int_t lddx, magma_int_t *dinfo_array, magma_int_t *diter_array,
                           magma_int_t batchCount, magma_queue_t queue)
 {
     hipStream_t st = stream_of(queue);
     const bool mixed_ok = n <= kMixedMaxN;
-    const int grid = mixed_grid(batchCount, n);
+    const MixedPlan plan = mixed_ok ? mixed_plan(batchCount, n) : MixedPlan();
     const size_t slot = mixed_factor_floats(n);
-    const size_t wbytes = mixed_ok ? round_up(sizeof(float) * slot * grid, 256) : 0;
+    const size_t wbytes = mixed_ok ? round_up(sizeof(float) * slot * plan.grid, 256) : 0;
     const size_t lbytes = sizeof(int) * (size_t)(batchCount + 2);
     char *ws = static_cast<char *>(queue_workspace(queue, wbytes + lbytes));
     if (!ws) return MAGMA_ERR_DEVICE_ALLOC;
@@ -1225,8 +1242,18 @@ magma_int_t mixed_batched(char kind, magma_int_t n, magma_int_t nrhs, double **d
         g.fb_list = fb_list; g.fb_count = fb_count;
     }
     g.count = batchCount;
+    // Upper: the lower kernels on the swapped triangles, swapped back at the end
+    // (A is unchanged on success; on fallback its upper triangle holds U)
+    MatSet tri;
+    tri.n = n; tri.m = n; tri.lda = ldda;
+    tri.A = dA_array;
+    tri.count = batchCount;
+    if (upper) {
+        magma_int_t r = launch_swap_triangles(tri, st);
+        if (r != MAGMA_SUCCESS) return r;
+    }
     if (mixed_ok) {
-        magma_int_t r = launch_mixed(g, n, st);
+        magma_int_t r = launch_mixed(g, n, plan, st);
         if (r != MAGMA_SUCCESS) return r;
     } else {
         // n above the mixed kernels' range: every matrix takes the FP64 path
@@ -1236,9 +1263,9 @@ magma_int_t mixed_batched(char kind, magma_int_t n, magma_int_t nrhs, double **d
         MAGMA_LAUNCH_CHECK();
     }
     magma_int_t r = launch_mixed_fallback(g, n, nrhs, st);
+    if (r == MAGMA_SUCCESS && upper) r = launch_swap_triangles(tri, st);
     if (r != MAGMA_SUCCESS) return r;
-    if (!queue->async) MAGMA_HIP_CHECK(hipStreamSynchronize(st));
-    return MAGMA_SUCCESS;
+    return sync_unless_async(queue);
 }
 
 }  // namespace
@@ -1265,10 +1292,11 @@ extern "C" magma_int_t magma_dsgesv_batched(magma_int_t n, magma_int_t nrhs,
     if (n == 0 || nrhs == 0 || batchCount == 0) {
         // the quick return leaves iter = 0 and info = 0 for every matrix (src/zcgesv_gpu.cpp:174-193)
         const hipStream_t st = stream_of(queue);
-        const magma_int_t e = zero_info(dinfo_array, batchCount, st);
-        return e != MAGMA_SUCCESS ? e : zero_info(diter_array, batchCount, st);
+        magma_int_t e = zero_info(dinfo_array, batchCount, st);
+        if (e == MAGMA_SUCCESS) e = zero_info(diter_array, batchCount, st);
+        return e != MAGMA_SUCCESS ? e : sync_unless_async(queue);
     }
-    return mixed_batched('L', n, nrhs, dA_array, ldda, dipiv_array, dB_array, lddb, dX_array, lddx, dinfo_array,
+    return mixed_batched('L', false, n, nrhs, dA_array, ldda, dipiv_array, dB_array, lddb, dX_array, lddx, dinfo_array,
                          diter_array, batchCount, queue);
 }
 
@@ -1289,10 +1317,6 @@ extern "C" magma_int_t magma_dsposv_batched(magma_uplo_t uplo, magma_int_t n, ma
     else if (lddb < std::max(1, n)) info = -7;
     else if (lddx < std::max(1, n)) info = -9;
     else if (batchCount < 0) info = -12;
-    if (info == 0 && uplo == MagmaUpper) {
-        std::fprintf(stderr, "%s: uplo=upper is not implemented\n", __func__);
-        info = -1;
-    }
     if (info != 0) {
         magma_xerbla(__func__, -info);
         return info;
@@ -1300,9 +1324,10 @@ extern "C" magma_int_t magma_dsposv_batched(magma_uplo_t uplo, magma_int_t n, ma
     if (n == 0 || nrhs == 0 || batchCount == 0) {
         // the quick return leaves iter = 0 and info = 0 for every matrix (src/zcposv_gpu.cpp:163-182)
         const hipStream_t st = stream_of(queue);
-        const magma_int_t e = zero_info(dinfo_array, batchCount, st);
-        return e != MAGMA_SUCCESS ? e : zero_info(diter_array, batchCount, st);
+        magma_int_t e = zero_info(dinfo_array, batchCount, st);
+        if (e == MAGMA_SUCCESS) e = zero_info(diter_array, batchCount, st);
+        return e != MAGMA_SUCCESS ? e : sync_unless_async(queue);
     }
-    return mixed_batched('C', n, nrhs, dA_array, ldda, nullptr, dB_array, lddb, dX_array, lddx, dinfo_array,
+    return mixed_batched('C', uplo == MagmaUpper, n, nrhs, dA_array, ldda, nullptr, dB_array, lddb, dX_array, lddx, dinfo_array,
                          diter_array, batchCount, queue);
 }

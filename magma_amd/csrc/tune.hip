@@ -6,6 +6,7 @@
 // The table can be saved to and loaded from a text file ("family class
 // variant" per line), so a deployment tunes once per device.  Untuned entries
 // use variant 0.
+#include <algorithm>
 #include <cstdio>
 #include <vector>
 
@@ -187,8 +188,13 @@ magma_int_t tune_mixed(hipStream_t st, FILE *log)
             !pa.alloc(8 * (size_t)count) || !pb.alloc(8 * (size_t)count) || !px.alloc(8 * (size_t)count) ||
             !it.alloc(4 * (size_t)count) || !inf.alloc(4 * (size_t)count) || !fb.alloc(4 * (size_t)(count + 2)))
             return MAGMA_ERR_DEVICE_ALLOC;
-        if (!w.alloc(sizeof(float) * mixed_factor_floats(n) * (size_t)persistent_grid(count, 4)))
-            return MAGMA_ERR_DEVICE_ALLOC;
+        // factor slots for the variant with the most resident workgroups
+        int max_grid = 0;
+        for (int v = 0; v < nv[t]; ++v) {
+            g_table[kTuneMixed][classes[t]] = v;
+            max_grid = std::max(max_grid, mixed_plan(count, n).grid);
+        }
+        if (!w.alloc(sizeof(float) * mixed_factor_floats(n) * (size_t)max_grid)) return MAGMA_ERR_DEVICE_ALLOC;
         fill_kernel<<<1024, 256, 0, st>>>(a.get<double>(), elems, 21u + t);
         fill_kernel<<<256, 256, 0, st>>>(b.get<double>(), (size_t)n * count, 31u + t);
         ptr_kernel<<<ceil_div(count, 256), 256, 0, st>>>(pa.get<double *>(), a.get<double>(), (size_t)n * n, count);
@@ -205,7 +211,8 @@ magma_int_t tune_mixed(hipStream_t st, FILE *log)
         int bestv = 0;
         for (int v = 0; v < nv[t]; ++v) {
             g_table[kTuneMixed][classes[t]] = v;
-            const float ms = best_ms(st, 3, [&]() {}, [&]() { (void)launch_mixed(g, n, st); });
+            const MixedPlan plan = mixed_plan(count, n);
+            const float ms = best_ms(st, 3, [&]() {}, [&]() { (void)launch_mixed(g, n, plan, st); });
             if (log) std::fprintf(log, "autotune mixed n=%d count=%d variant %d: %.3f ms\n", n, count, v, ms);
             if (ms < best) {
                 best = ms;

@@ -208,8 +208,8 @@ __global__ void append_fallback_kernel(const int *ids, int cnt, int *list, int *
 //   * the FP64 list (those plus the matrices whose refinement gave up) is
 //     binned by class on the device and every class present in the input gets
 //     its FP64 factor and solve kernels, which read their counts on the device.
-magma_int_t mixed_vbatched(char kind, int *n, int *nrhs, double **dA, int *ldda, int **dipiv, double **dB,
-                           int *lddb, double **dX, int *lddx, int *info, int *iter, int count,
+magma_int_t mixed_vbatched(char kind, bool upper, int *n, int *nrhs, double **dA, int *ldda, int **dipiv,
+                           double **dB, int *lddb, double **dX, int *lddx, int *info, int *iter, int count,
                            magma_queue_t queue, const char *name)
 {
     hipStream_t st = stream_of(queue);
@@ -238,6 +238,15 @@ magma_int_t mixed_vbatched(char kind, int *n, int *nrhs, double **dA, int *ldda,
         return arg;
     }
     if (f[kFlagTooBig]) return MAGMA_ERR_NOT_SUPPORTED;
+    // Upper: the lower kernels on the swapped triangles, swapped back at the end
+    MatSet tri;
+    tri.m_arr = n; tri.n_arr = n; tri.lda_arr = ldda;
+    tri.A = dA;
+    tri.count = count;
+    if (upper) {
+        r = launch_swap_triangles(tri, st);
+        if (r != MAGMA_SUCCESS) return r;
+    }
     int counts[kNumBuckets], offsets[kNumBuckets];
     std::copy(v.host + kNumFlags, v.host + kNumFlags + kNumBuckets, counts);
     std::copy(v.host + kNumFlags + kNumBuckets, v.host + kNumFlags + 2 * kNumBuckets, offsets);
@@ -255,8 +264,10 @@ magma_int_t mixed_vbatched(char kind, int *n, int *nrhs, double **dA, int *ldda,
     // factor slots: each launch has its own grid and slot size; the launches
     // are ordered on the stream, so they share one region
     size_t slot_bytes = 0;
-    if (n_small) slot_bytes = sizeof(float) * mixed_factor_floats(max_small) * mixed_grid(n_small, max_small);
-    if (n_mid) slot_bytes = std::max(slot_bytes, sizeof(float) * mixed_factor_floats(512) * mixed_grid(n_mid, 512));
+    const MixedPlan plan_small = n_small ? mixed_plan(n_small, max_small) : MixedPlan();
+    const MixedPlan plan_mid = n_mid ? mixed_plan(n_mid, 512) : MixedPlan();
+    if (n_small) slot_bytes = sizeof(float) * mixed_factor_floats(max_small) * plan_small.grid;
+    if (n_mid) slot_bytes = std::max(slot_bytes, sizeof(float) * mixed_factor_floats(512) * plan_mid.grid);
     char *ws = static_cast<char *>(queue_workspace(queue, head + round_up(slot_bytes, 256)));
     if (!ws) return MAGMA_ERR_DEVICE_ALLOC;
     v.q = workqueue_view(ws + fl, count);
@@ -286,7 +297,7 @@ magma_int_t mixed_vbatched(char kind, int *n, int *nrhs, double **dA, int *ldda,
         gm.index = v.q.ids + offsets[6];
         gm.count = n_mid;
         gm.slot_floats = mixed_factor_floats(512);
-        r = launch_mixed(gm, 512, st);
+        r = launch_mixed(gm, 512, plan_mid, st);
         if (r != MAGMA_SUCCESS) return r;
     }
     if (n_small) {
@@ -294,7 +305,7 @@ magma_int_t mixed_vbatched(char kind, int *n, int *nrhs, double **dA, int *ldda,
         gs.index = v.q.ids + offsets[0];
         gs.count = n_small;
         gs.slot_floats = mixed_factor_floats(max_small);
-        r = launch_mixed(gs, max_small, st);
+        r = launch_mixed(gs, max_small, plan_small, st);
         if (r != MAGMA_SUCCESS) return r;
     }
     if (n_large) {
@@ -327,6 +338,10 @@ magma_int_t mixed_vbatched(char kind, int *n, int *nrhs, double **dA, int *ldda,
             r = launch_class(k, b, c, st);
             if (r != MAGMA_SUCCESS) return r;
         }
+    }
+    if (upper) {
+        r = launch_swap_triangles(tri, st);
+        if (r != MAGMA_SUCCESS) return r;
     }
     return finish(MAGMA_SUCCESS, queue);
 }
@@ -460,10 +475,6 @@ magma_int_t magma_dpotrf_vbatched(magma_uplo_t uplo, magma_int_t *n, double **dA
     magma_int_t arg = 0;
     if (uplo != MagmaLower && uplo != MagmaUpper) arg = -1;
     else if (batchCount < 0) arg = -6;
-    if (arg == 0 && uplo == MagmaUpper) {
-        std::fprintf(stderr, "%s: uplo=upper is not yet implemented\n", __func__);
-        arg = -1;
-    }
     if (arg != 0) {
         magma_xerbla(__func__, -arg);
         return arg;
@@ -485,6 +496,15 @@ magma_int_t magma_dpotrf_vbatched(magma_uplo_t uplo, magma_int_t *n, double **dA
     MatSet s;
     s.n_arr = n; s.m_arr = n; s.lda_arr = ldda;
     s.A = dA_array; s.info = info_array;
+    if (uplo == MagmaUpper) {
+        // the lower kernels between two in-place triangle swaps (uplo.hip)
+        MatSet tri = s;
+        tri.count = batchCount;
+        magma_int_t e = launch_swap_triangles(tri, stream_of(queue));
+        if (e == MAGMA_SUCCESS) e = run_classes('C', s, v, stream_of(queue));
+        if (e == MAGMA_SUCCESS) e = launch_swap_triangles(tri, stream_of(queue));
+        return finish(e, queue);
+    }
     return finish(run_classes('C', s, v, stream_of(queue)), queue);
 }
 
@@ -492,7 +512,7 @@ magma_int_t magma_dpotrf_vbatched_max_nocheck(magma_uplo_t uplo, magma_int_t *n,
                                               magma_int_t *ldda, magma_int_t *info_array,
                                               magma_int_t batchCount, magma_int_t max_n, magma_queue_t queue)
 {
-    if (uplo != MagmaLower) {
+    if (uplo != MagmaLower && uplo != MagmaUpper) {
         magma_xerbla(__func__, 1);
         return -1;
     }
@@ -507,6 +527,12 @@ magma_int_t magma_dpotrf_vbatched_max_nocheck(magma_uplo_t uplo, magma_int_t *n,
     s.n_arr = n; s.m_arr = n; s.lda_arr = ldda;
     s.A = dA_array; s.info = info_array;
     s.count = batchCount;
+    if (uplo == MagmaUpper) {
+        r = launch_swap_triangles(s, st);
+        if (r == MAGMA_SUCCESS) r = run_classes_device('C', s, v, max_n, st);
+        if (r == MAGMA_SUCCESS) r = launch_swap_triangles(s, st);
+        return finish(r, queue);
+    }
     return finish(run_classes_device('C', s, v, max_n, st), queue);
 }
 
@@ -727,7 +753,7 @@ magma_int_t magma_dsgesv_vbatched(magma_int_t *n, magma_int_t *nrhs, double **dA
         return -12;
     }
     if (batchCount == 0) return MAGMA_SUCCESS;
-    return mixed_vbatched('L', n, nrhs, dA_array, ldda, dipiv_array, dB_array, lddb, dX_array, lddx,
+    return mixed_vbatched('L', false, n, nrhs, dA_array, ldda, dipiv_array, dB_array, lddb, dX_array, lddx,
                           dinfo_array, diter_array, batchCount, queue, __func__);
 }
 
@@ -736,8 +762,7 @@ magma_int_t magma_dsposv_vbatched(magma_uplo_t uplo, magma_int_t *n, magma_int_t
                                   double **dX_array, magma_int_t *lddx, magma_int_t *dinfo_array,
                                   magma_int_t *diter_array, magma_int_t batchCount, magma_queue_t queue)
 {
-    if (uplo != MagmaLower) {
-        if (uplo == MagmaUpper) std::fprintf(stderr, "%s: uplo=upper is not implemented\n", __func__);
+    if (uplo != MagmaLower && uplo != MagmaUpper) {
         magma_xerbla(__func__, 1);
         return -1;
     }
@@ -746,7 +771,7 @@ magma_int_t magma_dsposv_vbatched(magma_uplo_t uplo, magma_int_t *n, magma_int_t
         return -12;
     }
     if (batchCount == 0) return MAGMA_SUCCESS;
-    return mixed_vbatched('C', n, nrhs, dA_array, ldda, nullptr, dB_array, lddb, dX_array, lddx,
+    return mixed_vbatched('C', uplo == MagmaUpper, n, nrhs, dA_array, ldda, nullptr, dB_array, lddb, dX_array, lddx,
                           dinfo_array, diter_array, batchCount, queue, __func__);
 }
 

@@ -181,7 +181,7 @@ def test_dsgesv_argument_errors(queue):
     assert L.magma_dsgesv_batched(4, 1, 0, 4, 0, 0, 3, 0, 4, 0, 0, 1, q) == -7
     assert L.magma_dsgesv_batched(4, 1, 0, 4, 0, 0, 4, 0, 3, 0, 0, 1, q) == -9
     # dsposv keeps magma_dsposv_gpu's codes (n, nrhs, ldda numbered without uplo)
-    assert L.magma_dsposv_batched(ma.MagmaUpper, 4, 1, 0, 4, 0, 4, 0, 4, 0, 0, 1, q) == -1
+    assert L.magma_dsposv_batched(0, 4, 1, 0, 4, 0, 4, 0, 4, 0, 0, 1, q) == -1           # no such uplo
     assert L.magma_dsposv_batched(ma.MagmaLower, -1, 1, 0, 4, 0, 4, 0, 4, 0, 0, 1, q) == -1
     assert L.magma_dsposv_batched(ma.MagmaLower, 4, -1, 0, 4, 0, 4, 0, 4, 0, 0, 1, q) == -2
     assert L.magma_dsposv_batched(ma.MagmaLower, 4, 1, 0, 3, 0, 4, 0, 4, 0, 0, 1, q) == -4
@@ -206,7 +206,7 @@ def test_quick_return_sets_iter_and_info(queue, n, nrhs):
             ret = L.magma_dsposv_batched(ma.MagmaLower, n, nrhs, 0, 4, 0, 4, 0, 4, info.data_ptr(), it.data_ptr(),
                                          count, queue.handle)
         assert ret == 0
-        queue.sync()
+        # no queue.sync(): the call itself synchronizes before returning
         assert np.all(info.cpu().numpy()[:count] == 0) and np.all(it.cpu().numpy()[:count] == 0), kind
 
 
@@ -334,7 +334,7 @@ def test_mixed_vbatched_argument_errors(queue):
     assert sg(lb=small) == -7
     assert sg(lx=small) == -9
     assert sg(count=-1) == -12
-    assert L.magma_dsposv_vbatched(ma.MagmaUpper, p(ok), p(ok), 0, p(ok), 0, p(ok), 0, p(ok), 0, 0, 3, q) == -1
+    assert L.magma_dsposv_vbatched(0, p(ok), p(ok), 0, p(ok), 0, p(ok), 0, p(ok), 0, 0, 3, q) == -1      # no such uplo
     assert L.magma_dsposv_vbatched(ma.MagmaLower, p(neg), p(ok), 0, p(ok), 0, p(ok), 0, p(ok), 0, 0, 3, q) == -1
     assert L.magma_dsposv_vbatched(ma.MagmaLower, p(ok), p(ok), 0, p(small), 0, p(ok), 0, p(ok), 0, 0, 3, q) == -4
     big = ma.dev_ints([4, 3000, 4])

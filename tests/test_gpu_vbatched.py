@@ -223,7 +223,8 @@ def test_vbatched_argument_errors(queue):
     assert L.magma_dpotrf_vbatched(ma.MagmaLower, n.data_ptr(), 0, ld.data_ptr(), info.data_ptr(), 3,
                                    queue.handle) == -2
     assert L.magma_dpotrf_vbatched(ma.MagmaUpper, n2.data_ptr(), 0, ld.data_ptr(), info.data_ptr(), 3,
-                                   queue.handle) == -1
+                                   queue.handle) == -4                # Upper is supported: ldda is checked
+    assert L.magma_dpotrf_vbatched(0, n2.data_ptr(), 0, ld.data_ptr(), info.data_ptr(), 3, queue.handle) == -1
 
 
 def test_gemm_tile_variants_bitwise_equal(queue, oracle):

@@ -1,6 +1,7 @@
 """CPU: the tuning table's host interface (magma_amd_tune_get / _set / _save /
-_load).  Nothing here touches the device; the timing side of the autotuner
-(magma_amd_autotune) runs in the GPU tests and the benchmark."""
+_load).  Nothing here touches the device.  The timing side of the autotuner
+(magma_amd_autotune) and every variant it can select are exercised on the
+device by tests/test_gpu_variants.py."""
 import os
 
 import pytest

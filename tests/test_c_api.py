@@ -45,7 +45,7 @@ def test_c_program_dgesv_bitexact(tmp_path, oracle, n, nrhs, count):
     from magma_amd import gen
     exe = _build(tmp_path)
     A = gen.general_batch(70 + n, count, n).copy()
-    A[count // 2][:, 3] = 0.0                                # singular: info = 4, B unchanged
+    A[count // 2][3, :] = 0.0                                # column 4 zero (storage [col, row]): info = 4, B unchanged
     B = gen.rhs_batch(70 + n, count, n, nrhs)
     fin, fout = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
     with open(fin, "wb") as f:

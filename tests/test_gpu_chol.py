@@ -193,7 +193,10 @@ def test_potrf_upper_info(queue, oracle, n):
     for s in range(count):
         Uo, info_o = oracle.dpotrf(A[s], uplo=b"U")
         assert info.cpu().numpy()[s] == info_o, s
-        assert same(upper(U[s]), upper(Uo)), s
+        # after a failed column only the leading (info-1) x (info-1) block is
+        # final (LAPACK leaves the rest unspecified), as in test_potrf_info
+        k = info_o - 1 if info_o > 0 else n
+        assert same(upper(U[s])[:k, :k], upper(Uo)[:k, :k]), s
 
 
 @pytest.mark.parametrize("n,nrhs", [(16, 3), (64, 4), (100, 2), (300, 1)])
@@ -213,7 +216,8 @@ def test_posv_upper(queue, oracle, n, nrhs):
     for s in range(count):
         Uo, info_o = oracle.dpotrf(A[s], uplo=b"U")
         assert inf[s] == info_o
-        assert same(upper(U[s]), upper(Uo))
+        k = info_o - 1 if info_o > 0 else n
+        assert same(upper(U[s])[:k, :k], upper(Uo)[:k, :k])
         if info_o:
             assert np.array_equal(X[s], B[s])
             continue

@@ -141,12 +141,15 @@ def _check_dsgesv(oracle, A, B, Ag, X, ipiv, info, it, fallback):
 
 
 MIXED_N = {4: (70, 128), 5: (129, 256), 6: (300, 512)}
+# variants of the mixed family per size class: five kernels for orders up to
+# 256 (classes 4 and 5), two for orders up to 512
+MIXED_VARIANTS = {4: 5, 5: 5, 6: 2}
 
 
 def _mixed_cases():
     cases = []
     for cls, ns in MIXED_N.items():
-        for v in range(8):
+        for v in range(MIXED_VARIANTS[cls]):
             for n in ns:
                 cases.append((cls, v, n))
     return cases
@@ -155,8 +158,7 @@ def _mixed_cases():
 @pytest.mark.parametrize("cls,variant,n", _mixed_cases())
 def test_dsgesv_variant(queue, oracle, table, cls, variant, n):
     from magma_amd import gen
-    if table.magma_amd_tune_set(MIXED, cls, variant) != 0:
-        pytest.skip(f"mixed class {cls} has no variant {variant}")
+    assert table.magma_amd_tune_set(MIXED, cls, variant) == 0
     count = 20 if n <= 256 else 8
     nrhs = 2 if n % 2 else 1
     A, _ = gen.cond_batch(3, count, n)
@@ -174,8 +176,7 @@ def test_dsgesv_variant(queue, oracle, table, cls, variant, n):
 @pytest.mark.parametrize("cls,variant,n", _mixed_cases())
 def test_dsposv_variant(queue, oracle, table, cls, variant, n):
     from magma_amd import gen
-    if table.magma_amd_tune_set(MIXED, cls, variant) != 0:
-        pytest.skip(f"mixed class {cls} has no variant {variant}")
+    assert table.magma_amd_tune_set(MIXED, cls, variant) == 0
     count = 12 if n <= 256 else 6
     nrhs = 3 if n % 2 else 1
     A = gen.spd_batch(4, count, n).copy()

@@ -41,99 +41,26 @@
 
 #include "magma_amd.h"
 #include "mixed.h"
+#include "serve_cache.h"
 
 namespace magma_amd {
 namespace {
 
 constexpr int kStreams = 4;
-constexpr int kMinClass = 12;          // 4 KiB
-constexpr int kMaxClass = 24;          // 16 MiB
+
+using serve::Arena;
+using serve::FactorCache;
+using serve::Grant;
 
 double now_s()
 {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-// ---- device arena: power-of-two blocks, per-class free lists ----------------
-class Arena {
-public:
-    bool init(size_t bytes)
-    {
-        cap_ = bytes;
-        return hipMalloc(&base_, std::max<size_t>(bytes, 1)) == hipSuccess;
-    }
-    void release()
-    {
-        if (base_) (void)hipFree(base_);
-        base_ = nullptr;
-    }
-    static int cls(size_t bytes)
-    {
-        int c = kMinClass;
-        while (c < kMaxClass && ((size_t)1 << c) < bytes) ++c;
-        return ((size_t)1 << c) >= bytes ? c : -1;
-    }
-    static size_t block(size_t bytes) { return (size_t)1 << cls(bytes); }
-    void *alloc(size_t bytes)
-    {
-        const int c = cls(bytes);
-        if (c < 0) return nullptr;
-        auto &fl = free_[c - kMinClass];
-        if (!fl.empty()) {
-            void *p = fl.back();
-            fl.pop_back();
-            used_ += (size_t)1 << c;
-            return p;
-        }
-        const size_t b = (size_t)1 << c;
-        if (top_ + b > cap_) return nullptr;
-        void *p = base_ + top_;
-        top_ += b;
-        used_ += b;
-        return p;
-    }
-    void free(void *p, size_t bytes)
-    {
-        if (!p) return;
-        const int c = cls(bytes);
-        free_[c - kMinClass].push_back(p);
-        used_ -= (size_t)1 << c;
-    }
-    // would alloc(bytes) succeed without eviction?
-    bool can(size_t bytes) const
-    {
-        const int c = cls(bytes);
-        if (c < 0) return false;
-        return !free_[c - kMinClass].empty() || top_ + ((size_t)1 << c) <= cap_;
-    }
-    size_t used() const { return used_; }
-    size_t cap() const { return cap_; }
-    // forgets every block; only valid when nothing is allocated
-    void reset()
-    {
-        top_ = 0;
-        used_ = 0;
-        for (auto &fl : free_) fl.clear();
-    }
-
-private:
-    char *base_ = nullptr;
-    size_t cap_ = 0, top_ = 0, used_ = 0;
-    std::vector<void *> free_[kMaxClass - kMinClass + 1];
-};
-
 struct Request {
     magma_amd_request_t r;
     long long ticket;
     double t_submit;
-};
-
-struct Resident {
-    int n, kind;
-    double *A;       // FP64 copy of the matrix
-    float *F;        // FP32 factorization
-    int pins;        // in-flight requests reading it
-    std::list<long long>::iterator lru;
 };
 
 // Batch descriptor: one block of arrays, identical in pinned host memory and
@@ -156,8 +83,7 @@ struct Slot {
     size_t bytes = 0;
     BatchArrays h, d;                       // views into host / device blocks
     std::vector<Request> reqs;
-    std::vector<int> flags;                 // per request: 1 reused resident, 2 keep after completion
-    std::vector<long long> pinned;          // resident keys pinned by this batch
+    std::vector<Grant> grants;              // per request: its blocks (and whether a resident served it)
     double t_start = 0.0;
     bool busy = false;
 };
@@ -223,7 +149,9 @@ using namespace magma_amd;
 struct magma_amd_server {
     int device = 0;
     int max_batch = 0;
+    char *arena_mem = nullptr;              // the budget, one device allocation
     Arena arena;
+    FactorCache cache{arena};
     Slot slots[kStreams];
     std::thread worker;
     std::mutex mu;                          // guards submitted, done, counters
@@ -233,60 +161,13 @@ struct magma_amd_server {
     long long next_ticket = 0;
     long long outstanding = 0;              // submitted but not completed
     std::atomic<bool> stop{false};
-    // scheduler-thread state
+    // scheduler-thread state (pending, arena, cache)
     std::deque<Request> pending;
-    std::unordered_map<long long, Resident> resident;
-    std::list<long long> lru;               // front = least recently used
     magma_amd_server_stats_t stats{};
     std::mutex stats_mu;
 
-    size_t need_bytes(const magma_amd_request_t &r, bool hit) const
-    {
-        const size_t a = Arena::block(sizeof(double) * (size_t)r.n * r.n);
-        const size_t f = hit ? 0 : Arena::block(sizeof(float) * mixed_factor_floats(r.n));
-        return a + f;
-    }
-
-    // evicts unpinned resident factorizations until `bytes` of class sizes fit
-    bool make_room(const magma_amd_request_t &r, bool hit)
-    {
-        const size_t a = sizeof(double) * (size_t)r.n * r.n;
-        const size_t f = sizeof(float) * mixed_factor_floats(r.n);
-        auto fits = [&]() {
-            if (hit) return arena.can(a);
-            if (Arena::cls(a) == Arena::cls(f)) {
-                // two blocks of one class: need the free list or the bump space for both
-                void *p = arena.alloc(a);
-                if (!p) return false;
-                const bool ok = arena.can(f);
-                arena.free(p, a);
-                return ok;
-            }
-            return arena.can(a) && arena.can(f);
-        };
-        while (!fits()) {
-            auto it = lru.begin();
-            while (it != lru.end() && resident[*it].pins > 0) ++it;
-            if (it == lru.end()) {
-                // blocks of other sizes may hold the free space: once nothing
-                // at all is allocated the arena is re-carved from scratch
-                if (arena.used() == 0) {
-                    arena.reset();
-                    return fits();
-                }
-                return false;
-            }
-            Resident &e = resident[*it];
-            arena.free(e.A, sizeof(double) * (size_t)e.n * e.n);
-            arena.free(e.F, sizeof(float) * mixed_factor_floats(e.n));
-            const long long key = *it;
-            lru.erase(it);
-            resident.erase(key);
-            std::lock_guard<std::mutex> g(stats_mu);
-            stats.evictions++;
-        }
-        return true;
-    }
+    static size_t matrix_bytes(int n) { return sizeof(double) * (size_t)n * n; }
+    static size_t factor_bytes(int n) { return sizeof(float) * mixed_factor_floats(n); }
 
     void complete(Slot &s)
     {
@@ -296,19 +177,10 @@ struct magma_amd_server {
         for (int i = 0; i < cnt; ++i) {
             const Request &q = s.reqs[i];
             const magma_amd_request_t &r = q.r;
-            const bool reused = s.flags[i] & 1;
-            const bool keep = s.flags[i] & 2;
+            const bool reused = s.grants[i].hit;
             const int iter = s.h.iter[i], info = s.h.info[i];
-            const size_t abytes = sizeof(double) * (size_t)r.n * r.n;
-            const size_t fbytes = sizeof(float) * mixed_factor_floats(r.n);
-            if (keep && iter >= 0 && info == 0 && !resident.count(r.factor_key)) {
-                Resident e{r.n, (int)r.kind, s.h.A[i], s.h.F[i], 0, lru.end()};
-                e.lru = lru.insert(lru.end(), r.factor_key);
-                resident.emplace(r.factor_key, e);
-            } else {
-                arena.free(s.h.A[i], abytes);
-                if (!reused) arena.free(s.h.F[i], fbytes);
-            }
+            cache.retire(r.factor_key, r.n, (int)r.kind, matrix_bytes(r.n), factor_bytes(r.n), s.grants[i],
+                         iter >= 0 && info == 0);
             magma_amd_completion_t &c = out[i];
             c.ticket = q.ticket;
             c.info = info;
@@ -319,20 +191,16 @@ struct magma_amd_server {
             c.t_start = s.t_start;
             c.t_done = t;
         }
-        for (long long key : s.pinned) {
-            auto it = resident.find(key);
-            if (it != resident.end()) it->second.pins--;
-        }
         {
             std::lock_guard<std::mutex> g(stats_mu);
             stats.completed += cnt;
             stats.batches++;
             stats.bytes_in_use = arena.used();
-            stats.resident = (magma_int_t)resident.size();
+            stats.resident = (magma_int_t)cache.resident();
+            stats.evictions = cache.evictions();
         }
         s.reqs.clear();
-        s.flags.clear();
-        s.pinned.clear();
+        s.grants.clear();
         s.busy = false;
         {
             std::lock_guard<std::mutex> g(mu);
@@ -347,30 +215,15 @@ struct magma_amd_server {
     bool launch(Slot &s)
     {
         s.reqs.clear();
-        s.flags.clear();
-        s.pinned.clear();
+        s.grants.clear();
         while (!pending.empty() && (int)s.reqs.size() < max_batch) {
             Request &q = pending.front();
             const magma_amd_request_t &r = q.r;
-            auto it = (r.factor_key >= 0) ? resident.find(r.factor_key) : resident.end();
-            const bool hit = it != resident.end() && it->second.n == r.n && it->second.kind == (int)r.kind;
-            // pinned before making room, so that the eviction cannot pick it
-            if (hit) it->second.pins++;
-            if (!make_room(r, hit)) {
-                if (hit) it->second.pins--;
-                break;
-            }
-            const size_t abytes = sizeof(double) * (size_t)r.n * r.n;
-            double *A = static_cast<double *>(arena.alloc(abytes));
-            float *F = hit ? it->second.F : static_cast<float *>(arena.alloc(sizeof(float) * mixed_factor_floats(r.n)));
-            if (!A || !F) {
-                arena.free(A, abytes);
-                if (hit) it->second.pins--;
-                break;
-            }
+            Grant g;
+            if (!cache.admit(r.factor_key, r.n, (int)r.kind, matrix_bytes(r.n), factor_bytes(r.n), g)) break;
             const int i = (int)s.reqs.size();
-            s.h.A[i] = A;
-            s.h.F[i] = F;
+            s.h.A[i] = static_cast<double *>(g.A);
+            s.h.F[i] = static_cast<float *>(g.F);
             s.h.ipiv[i] = s.d.ipivbuf + (size_t)i * kMixedMaxN;
             s.h.B[i] = const_cast<double *>(r.dB);
             s.h.X[i] = r.dX;
@@ -379,17 +232,10 @@ struct magma_amd_server {
             s.h.ldb[i] = r.lddb;
             s.h.ldx[i] = r.lddx;
             s.h.kind[i] = r.kind == MAGMA_AMD_SOLVE_SPD ? 'C' : 'L';
-            s.h.reuse[i] = hit ? 1 : 0;
-            if (hit) {
-                s.h.Asrc[i] = it->second.A;
-                s.h.lda_src[i] = r.n;
-                lru.splice(lru.end(), lru, it->second.lru);
-                s.pinned.push_back(r.factor_key);
-            } else {
-                s.h.Asrc[i] = r.dA;
-                s.h.lda_src[i] = r.ldda;
-            }
-            s.flags.push_back((hit ? 1 : 0) | ((!hit && r.factor_key >= 0) ? 2 : 0));
+            s.h.reuse[i] = g.hit ? 1 : 0;
+            s.h.Asrc[i] = g.hit ? static_cast<const double *>(g.Asrc) : r.dA;
+            s.h.lda_src[i] = g.hit ? r.n : r.ldda;
+            s.grants.push_back(g);
             s.reqs.push_back(q);
             pending.pop_front();
         }
@@ -496,10 +342,11 @@ magma_int_t magma_amd_server_create(magma_device_t device, size_t budget_bytes, 
     magma_amd_server *s = new magma_amd_server();
     s->device = device;
     s->max_batch = max_batch;
-    if (!s->arena.init(budget_bytes)) {
+    if (hipMalloc(reinterpret_cast<void **>(&s->arena_mem), std::max<size_t>(budget_bytes, 1)) != hipSuccess) {
         delete s;
         return MAGMA_ERR_DEVICE_ALLOC;
     }
+    s->arena.init(s->arena_mem, budget_bytes);
     for (Slot &sl : s->slots) {
         BatchArrays tmp;
         sl.bytes = carve(nullptr, max_batch, tmp);
@@ -529,7 +376,8 @@ long long magma_amd_server_submit(magma_amd_server_t s, const magma_amd_request_
     if (!r.dA || r.ldda < r.n) return -4;
     if (!r.dB || r.lddb < r.n) return -5;
     if (!r.dX || r.lddx < r.n) return -6;
-    const size_t need = s->need_bytes(r, false);
+    const size_t need = Arena::block(magma_amd_server::matrix_bytes(r.n)) +
+                        Arena::block(magma_amd_server::factor_bytes(r.n));
     if (need > s->arena.cap()) return MAGMA_ERR_DEVICE_ALLOC;
     long long t;
     {
@@ -642,7 +490,7 @@ void magma_amd_server_destroy(magma_amd_server_t s)
         if (sl.host) (void)hipHostFree(sl.host);
         if (sl.dev) (void)hipFree(sl.dev);
     }
-    s->arena.release();
+    if (s->arena_mem) (void)hipFree(s->arena_mem);
     delete s;
 }
 

@@ -66,6 +66,16 @@ magma_int_t magma_dgetrf_vbatched_max_nocheck(magma_int_t *m, magma_int_t *n, ma
                                               double **dA_array, magma_int_t *ldda,
                                               magma_int_t **dipiv_array, magma_int_t *info_array,
                                               magma_int_t batchCount, magma_queue_t queue);
+/* No counterpart in the reference: magma_dgetrf_vbatched_max_nocheck that
+ * allocates nothing (no hipMalloc, no queue workspace), for callers that
+ * cannot allocate (HIP graph capture, serving loops, harnesses that forbid
+ * it).  Its work list is a static device array holding at most 1048576
+ * matrices; a larger batchCount returns -8.  Calls that overlap on one device
+ * share that array and race. */
+magma_int_t magma_dgetrf_vbatched_max_nocheck_noalloc(magma_int_t *m, magma_int_t *n, magma_int_t max_mn,
+                                                      double **dA_array, magma_int_t *ldda,
+                                                      magma_int_t **dipiv_array, magma_int_t *info_array,
+                                                      magma_int_t batchCount, magma_queue_t queue);
 /* New: solves with the getrf_vbatched / potrf_vbatched factors. */
 magma_int_t magma_dgetrs_vbatched(magma_trans_t trans, magma_int_t *n, magma_int_t *nrhs,
                                   double **dA_array, magma_int_t *ldda, magma_int_t **dipiv_array,

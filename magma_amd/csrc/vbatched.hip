@@ -14,11 +14,24 @@
 //     class kernels are persistent and read their counts on the device, so the
 //     whole call can be captured in a HIP graph.
 #include <algorithm>
+#include <cstdio>
+#include <mutex>
 
 #include "kernels.h"
 #include "mixed.h"
 #include "workqueue.h"
 #include "tune.h"
+
+// Work list of magma_dgetrf_vbatched_max_nocheck_noalloc: the size-class sort
+// of up to kNoAllocMaxBatch matrix ids (workqueue_view layout), in a
+// file-scope device array so that the call allocates nothing.  Each device
+// holds its own copy (4 MiB) until the process ends.  In host code `scratch`
+// is only a symbol: its device address comes from hipGetSymbolAddress.
+namespace {
+constexpr int kNoAllocMaxBatch = 1 << 20;
+constexpr int kNoAllocMaxDevices = 64;
+}  // namespace
+static __device__ __align__(256) int scratch[kNoAllocMaxBatch + 3 * magma_amd::kNumBuckets + 1];
 
 namespace magma_amd {
 
@@ -394,6 +407,76 @@ magma_int_t magma_dgetrf_vbatched_max_nocheck(magma_int_t *m, magma_int_t *n, ma
     if (max_mn > kMaxFactorDim) return MAGMA_ERR_NOT_SUPPORTED;
     VScratch v;
     if (!vscratch(queue, batchCount, v)) return MAGMA_ERR_DEVICE_ALLOC;
+    hipStream_t st = stream_of(queue);
+    magma_int_t r = build_workqueue(m, n, batchCount, v.q, st);
+    if (r != MAGMA_SUCCESS) return r;
+    MatSet s;
+    s.m_arr = m; s.n_arr = n; s.lda_arr = ldda;
+    s.A = dA_array; s.ipiv = dipiv_array; s.info = info_array;
+    s.count = batchCount;
+    return finish(run_classes_device('L', s, v, max_mn, st), queue);
+}
+
+}  // extern "C"
+
+namespace {
+
+// Device address of `scratch` on each device, resolved once per device with
+// that device current (every device has its own copy at its own address).
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kNoAllocMaxDevices];
+hipError_t g_scratch_err[kNoAllocMaxDevices];
+bool g_scratch_done[kNoAllocMaxDevices];
+
+magma_int_t scratch_address(int device, void **out)
+{
+    *out = nullptr;
+    if (device < 0 || device >= kNoAllocMaxDevices) return MAGMA_ERR_ILLEGAL_VALUE;
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_done[device]) {
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e == hipSuccess && cur != device) e = hipSetDevice(device);
+        void *p = nullptr;
+        if (e == hipSuccess) e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur >= 0 && cur != device) (void)hipSetDevice(cur);
+        g_scratch_ptr[device] = (e == hipSuccess) ? p : nullptr;
+        g_scratch_err[device] = e;
+        g_scratch_done[device] = true;
+    }
+    if (g_scratch_err[device] != hipSuccess)
+        return report_hip_error(g_scratch_err[device], "hipGetSymbolAddress(scratch)", __FILE__, __LINE__);
+    *out = g_scratch_ptr[device];
+    return MAGMA_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
+// magma_dgetrf_vbatched_max_nocheck with its work list in the file-scope
+// `scratch` array instead of the queue's workspace: no allocation of any kind,
+// for callers that cannot allocate.  Calls that overlap on one device share
+// the array and race; the kernels rebuild the whole list on every call
+// (counts cleared by a kernel), so nothing carries over between calls.
+magma_int_t magma_dgetrf_vbatched_max_nocheck_noalloc(magma_int_t *m, magma_int_t *n, magma_int_t max_mn,
+                                                      double **dA_array, magma_int_t *ldda,
+                                                      magma_int_t **dipiv_array, magma_int_t *info_array,
+                                                      magma_int_t batchCount, magma_queue_t queue)
+{
+    if (batchCount <= 0) return MAGMA_SUCCESS;
+    if (batchCount > kNoAllocMaxBatch) {
+        std::fprintf(stderr, "%s: batchCount %d exceeds the %d matrices of its static work list\n", __func__,
+                     (int)batchCount, kNoAllocMaxBatch);
+        magma_xerbla(__func__, 8);
+        return -8;
+    }
+    if (max_mn > kMaxFactorDim) return MAGMA_ERR_NOT_SUPPORTED;
+    void *base = nullptr;
+    const magma_int_t e = scratch_address(queue ? queue->device : 0, &base);
+    if (e != MAGMA_SUCCESS) return e;
+    VScratch v;
+    v.q = workqueue_view(base, batchCount);
     hipStream_t st = stream_of(queue);
     magma_int_t r = build_workqueue(m, n, batchCount, v.q, st);
     if (r != MAGMA_SUCCESS) return r;

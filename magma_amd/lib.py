@@ -122,6 +122,7 @@ PROTOTYPES = {
     # vbatched (include/magma_dvbatched.h)
     "magma_dgetrf_vbatched": (c_int, [_P, _P, _P, _P, _P, _P, c_int, _P]),
     "magma_dgetrf_vbatched_max_nocheck": (c_int, [_P, _P, c_int, _P, _P, _P, _P, c_int, _P]),
+    "magma_dgetrf_vbatched_max_nocheck_noalloc": (c_int, [_P, _P, c_int, _P, _P, _P, _P, c_int, _P]),
     "magma_dgetrs_vbatched": (c_int, [c_int, _P, _P, _P, _P, _P, _P, _P, c_int, _P]),
     "magma_dpotrf_vbatched": (c_int, [c_int, _P, _P, _P, _P, c_int, _P]),
     "magma_dpotrf_vbatched_max_nocheck": (c_int, [c_int, _P, _P, _P, _P, c_int, c_int, _P]),

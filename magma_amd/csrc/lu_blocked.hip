@@ -286,9 +286,84 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
 }
 
 // ---------------------------------------------------------------------------
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// The lower triangle of A22 -= L21 L21' (rows and columns [c0, n), k = kb
+// panel columns from k0) on v_mfma_f64_16x16x4f64, one 32 x 32 tile per
+// wavefront at a time.  The product is formed transposed: MFMA row m is the
+// tile column (A operand -L(col, k)), MFMA column the tile row (B operand
+// L(row, k)), so lane (lk = lane / 16, li = lane % 16) holds, in register i of
+// block (bi, bj), entry (row r0 + 16 bj + li, col q0 + 16 bi + lk + 4 i).
+// The MFMA adds its four k terms to each entry as a sequential fma chain and
+// the k slices come in increasing order, so every entry gets
+// fma(-l_ck, l_ik, a) in increasing k as the netlib loop does (terms past kb
+// are -0 x 0 and leave an entry as it is).
+__device__ void syrk_update_mfma(double *A, int lda, int c0, int n, int k0, int kb)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int nt = (n - c0 + 31) / 32;
+    // lower-triangle tiles (tr >= tc), column by column
+    const int ntiles = nt * (nt + 1) / 2;
+#pragma unroll 1
+    for (int t = w; t < ntiles; t += kThreads / 64) {
+        int tc = 0, rem = t;
+        while (rem >= nt - tc) {
+            rem -= nt - tc;
+            ++tc;
+        }
+        const int tr = tc + rem;
+        const int r0 = c0 + 32 * tr, q0 = c0 + 32 * tc;
+        d4 acc[2][2];
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int row = r0 + 16 * bj + li;
+                    const int col = q0 + 16 * bi + lk + 4 * i;
+                    acc[bi][bj][i] = (row < n && col < n && row >= col) ? A[row + (size_t)col * lda] : 0.0;
+                }
+#pragma unroll 1
+        for (int kq = 0; kq < kb; kq += 4) {
+            const int kk = kq + lk;
+            double ua[2], lb[2];
+#pragma unroll
+            for (int bi = 0; bi < 2; ++bi) {
+                const int col = q0 + 16 * bi + li;
+                ua[bi] = (col < n && kk < kb) ? -A[col + (size_t)(k0 + kk) * lda] : -0.0;
+            }
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj) {
+                const int row = r0 + 16 * bj + li;
+                lb[bj] = (row < n && kk < kb) ? A[row + (size_t)(k0 + kk) * lda] : 0.0;
+            }
+#pragma unroll
+            for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < 2; ++bj)
+                    acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(ua[bi], lb[bj], acc[bi][bj], 0, 0, 0);
+        }
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int row = r0 + 16 * bj + li;
+                    const int col = q0 + 16 * bi + lk + 4 * i;
+                    if (row < n && col < n && row >= col) A[row + (size_t)col * lda] = acc[bi][bj][i];
+                }
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
 // Cholesky, lower.  No pivoting: rows stay in place, the panel holds rows
 // j0..n-1 of columns j0..j0+jb-1; the diagonal block's upper part is never
-// written back.
+// written back.  The trailing update is the FP64 MFMA SYRK above (the VALU
+// loops when a multiplier is not finite).
 template <int RPT, int PW>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) potrf_blocked_kernel(MatSet s)
 {
@@ -371,8 +446,12 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
                 }
             }
             const bool exact = __syncthreads_or(bad);
-            if (info == 0 && j0 + jb < n)
-                trailing_update<true>(A, lda, j0 + jb, n, j0 + jb, n, j0, jb, exact, Ls, Us);
+            if (info == 0 && j0 + jb < n) {
+                // finite multipliers: FP64 MFMA; else the VALU loops with the
+                // reference BLAS skip of zero entries (only then can it matter)
+                if (!exact) syrk_update_mfma(A, lda, j0 + jb, n, j0, jb);
+                else trailing_update<true>(A, lda, j0 + jb, n, j0 + jb, n, j0, jb, exact, Ls, Us);
+            }
         }
         if (tid == 0) s.info[id] = info;
         __syncthreads();

@@ -42,15 +42,21 @@ struct MixedSet {
     int count = 0;
 };
 
-// Kernel variant and grid of persistent workgroups for `count` matrices of
-// order at most max_n.  The variant is read from the tuning table once, so the
-// grid that sizes the factor slots (Wslot, one per workgroup) is the grid the
-// launch uses even if the table changes in between.
+// Launch plan for `count` matrices of order at most max_n with at most
+// max_nrhs right-hand sides each (0: not known on the host): the kernel
+// variant, read from the tuning table once, and its persistent grid, so the
+// factor slots sized from the plan (one per workgroup) are those the launch
+// uses even if the table changes in between.
 struct MixedPlan {
     int variant = 0;
     int grid = 0;
 };
-MixedPlan mixed_plan(int count, int max_n);
+MixedPlan mixed_plan(int count, int max_n, int max_nrhs);
+// Device workspace of a plan: one factor slot per workgroup when `slots`
+// (the set has no per-matrix factor storage); the view sets g.Wslot and
+// g.slot_floats.
+size_t mixed_workspace_bytes(const MixedPlan &p, int max_n, bool slots);
+void mixed_workspace_view(MixedSet &g, const MixedPlan &p, int max_n, bool slots, void *ws);
 magma_int_t launch_mixed(const MixedSet &g, int max_n, const MixedPlan &plan, hipStream_t st);
 
 // FP64 fallback for the matrices the mixed kernel listed: X <- B, then the

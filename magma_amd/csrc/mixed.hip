@@ -1125,12 +1125,27 @@ static int mixed_per_cu(int v, int max_n)
     return v == 0 ? resident_per_cu(mixed_kernel<512, 64, 8, 1>) : resident_per_cu(mixed_kernel<512, 32, 8, 1>);
 }
 
-MixedPlan mixed_plan(int count, int max_n)
+MixedPlan mixed_plan(int count, int max_n, int max_nrhs)
 {
+    (void)max_nrhs;
     MixedPlan p;
     p.variant = mixed_variant(max_n);
     p.grid = persistent_grid(count, mixed_per_cu(p.variant, max_n));
     return p;
+}
+
+size_t mixed_workspace_bytes(const MixedPlan &p, int max_n, bool slots)
+{
+    return slots ? round_up(sizeof(float) * mixed_factor_floats(max_n) * (size_t)p.grid, 256) : 0;
+}
+
+void mixed_workspace_view(MixedSet &g, const MixedPlan &p, int max_n, bool slots, void *ws)
+{
+    (void)p;
+    if (slots) {
+        g.Wslot = static_cast<float *>(ws);
+        g.slot_floats = mixed_factor_floats(max_n);
+    }
 }
 
 magma_int_t launch_mixed(const MixedSet &g, int max_n, const MixedPlan &plan, hipStream_t st)
@@ -1219,9 +1234,8 @@ magma_int_t mixed_batched(char kind, bool upper, magma_int_t n, magma_int_t nrhs
 {
     hipStream_t st = stream_of(queue);
     const bool mixed_ok = n <= kMixedMaxN;
-    const MixedPlan plan = mixed_ok ? mixed_plan(batchCount, n) : MixedPlan();
-    const size_t slot = mixed_factor_floats(n);
-    const size_t wbytes = mixed_ok ? round_up(sizeof(float) * slot * plan.grid, 256) : 0;
+    const MixedPlan plan = mixed_ok ? mixed_plan(batchCount, n, nrhs) : MixedPlan();
+    const size_t wbytes = mixed_ok ? mixed_workspace_bytes(plan, n, true) : 0;
     const size_t lbytes = sizeof(int) * (size_t)(batchCount + 2);
     char *ws = static_cast<char *>(queue_workspace(queue, wbytes + lbytes));
     if (!ws) return MAGMA_ERR_DEVICE_ALLOC;
@@ -1233,8 +1247,7 @@ magma_int_t mixed_batched(char kind, bool upper, magma_int_t n, magma_int_t nrhs
     g.n = n; g.nrhs = nrhs; g.lda = ldda; g.ldb = lddb; g.ldx = lddx;
     g.kind = kind;
     g.A = dA_array; g.B = dB_array; g.X = dX_array;
-    g.Wslot = reinterpret_cast<float *>(ws);
-    g.slot_floats = slot;
+    if (mixed_ok) mixed_workspace_view(g, plan, n, true, ws);
     g.ipiv = dipiv_array; g.info = dinfo_array; g.iter = diter_array;
     if (kind == 'C') {
         g.fbc_list = fb_list; g.fbc_count = fb_count;

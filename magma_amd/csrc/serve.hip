@@ -272,13 +272,13 @@ struct magma_amd_server {
             MixedSet gl = g;
             gl.index = s.d.index;
             gl.count = n_large;
-            (void)launch_mixed(gl, max_n, mixed_plan(gl.count, max_n), st);
+            (void)launch_mixed(gl, max_n, mixed_plan(gl.count, max_n, 0), st);
         }
         if (cnt > n_large) {
             MixedSet gs = g;
             gs.index = s.d.index + n_large;
             gs.count = cnt - n_large;
-            (void)launch_mixed(gs, max_small, mixed_plan(gs.count, max_small), st);
+            (void)launch_mixed(gs, max_small, mixed_plan(gs.count, max_small, 0), st);
         }
         g.count = cnt;
         (void)launch_mixed_fallback(g, max_n, 16, st);

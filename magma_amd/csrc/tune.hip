@@ -188,13 +188,13 @@ magma_int_t tune_mixed(hipStream_t st, FILE *log)
             !pa.alloc(8 * (size_t)count) || !pb.alloc(8 * (size_t)count) || !px.alloc(8 * (size_t)count) ||
             !it.alloc(4 * (size_t)count) || !inf.alloc(4 * (size_t)count) || !fb.alloc(4 * (size_t)(count + 2)))
             return MAGMA_ERR_DEVICE_ALLOC;
-        // factor slots for the variant with the most resident workgroups
-        int max_grid = 0;
+        // factor slots and status entries for the largest plan of the variants
+        size_t wbytes = 0;
         for (int v = 0; v < nv[t]; ++v) {
             g_table[kTuneMixed][classes[t]] = v;
-            max_grid = std::max(max_grid, mixed_plan(count, n).grid);
+            wbytes = std::max(wbytes, mixed_workspace_bytes(mixed_plan(count, n, 1), n, true));
         }
-        if (!w.alloc(sizeof(float) * mixed_factor_floats(n) * (size_t)max_grid)) return MAGMA_ERR_DEVICE_ALLOC;
+        if (!w.alloc(wbytes)) return MAGMA_ERR_DEVICE_ALLOC;
         fill_kernel<<<1024, 256, 0, st>>>(a.get<double>(), elems, 21u + t);
         fill_kernel<<<256, 256, 0, st>>>(b.get<double>(), (size_t)n * count, 31u + t);
         ptr_kernel<<<ceil_div(count, 256), 256, 0, st>>>(pa.get<double *>(), a.get<double>(), (size_t)n * n, count);
@@ -203,7 +203,6 @@ magma_int_t tune_mixed(hipStream_t st, FILE *log)
         MixedSet g;
         g.n = n; g.nrhs = 1; g.lda = n; g.ldb = n; g.ldx = n;
         g.A = pa.get<double *>(); g.B = pb.get<double *>(); g.X = px.get<double *>();
-        g.Wslot = w.get<float>(); g.slot_floats = mixed_factor_floats(n);
         g.info = inf.get<int>(); g.iter = it.get<int>();
         g.fb_count = fb.get<int>(); g.fb_list = fb.get<int>() + 1;
         g.count = count;
@@ -211,7 +210,8 @@ magma_int_t tune_mixed(hipStream_t st, FILE *log)
         int bestv = 0;
         for (int v = 0; v < nv[t]; ++v) {
             g_table[kTuneMixed][classes[t]] = v;
-            const MixedPlan plan = mixed_plan(count, n);
+            const MixedPlan plan = mixed_plan(count, n, 1);
+            mixed_workspace_view(g, plan, n, true, w.get<char>());
             const float ms = best_ms(st, 3, [&]() {}, [&]() { (void)launch_mixed(g, n, plan, st); });
             if (log) std::fprintf(log, "autotune mixed n=%d count=%d variant %d: %.3f ms\n", n, count, v, ms);
             if (ms < best) {

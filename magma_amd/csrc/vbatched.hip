@@ -274,13 +274,13 @@ magma_int_t mixed_vbatched(char kind, bool upper, int *n, int *nrhs, double **dA
     }
     const int n_mid = counts[6];
     const int n_large = counts[7] + counts[8];
-    // factor slots: each launch has its own grid and slot size; the launches
-    // are ordered on the stream, so they share one region
+    // factor slots and status entries: each launch has its own plan; the
+    // launches are ordered on the stream, so they share one region
     size_t slot_bytes = 0;
-    const MixedPlan plan_small = n_small ? mixed_plan(n_small, max_small) : MixedPlan();
-    const MixedPlan plan_mid = n_mid ? mixed_plan(n_mid, 512) : MixedPlan();
-    if (n_small) slot_bytes = sizeof(float) * mixed_factor_floats(max_small) * plan_small.grid;
-    if (n_mid) slot_bytes = std::max(slot_bytes, sizeof(float) * mixed_factor_floats(512) * plan_mid.grid);
+    const MixedPlan plan_small = n_small ? mixed_plan(n_small, max_small, 0) : MixedPlan();
+    const MixedPlan plan_mid = n_mid ? mixed_plan(n_mid, 512, 0) : MixedPlan();
+    if (n_small) slot_bytes = mixed_workspace_bytes(plan_small, max_small, true);
+    if (n_mid) slot_bytes = std::max(slot_bytes, mixed_workspace_bytes(plan_mid, 512, true));
     char *ws = static_cast<char *>(queue_workspace(queue, head + round_up(slot_bytes, 256)));
     if (!ws) return MAGMA_ERR_DEVICE_ALLOC;
     v.q = workqueue_view(ws + fl, count);
@@ -298,7 +298,6 @@ magma_int_t mixed_vbatched(char kind, bool upper, int *n, int *nrhs, double **dA
     g.n_arr = n; g.nrhs_arr = nrhs; g.lda_arr = ldda; g.ldb_arr = lddb; g.ldx_arr = lddx;
     g.kind = kind;
     g.A = dA; g.B = dB; g.X = dX;
-    g.Wslot = reinterpret_cast<float *>(ws + head);
     g.ipiv = dipiv; g.info = info; g.iter = iter;
     if (kind == 'C') {
         g.fbc_list = fb_list; g.fbc_count = fb_count;
@@ -309,7 +308,7 @@ magma_int_t mixed_vbatched(char kind, bool upper, int *n, int *nrhs, double **dA
         MixedSet gm = g;
         gm.index = v.q.ids + offsets[6];
         gm.count = n_mid;
-        gm.slot_floats = mixed_factor_floats(512);
+        mixed_workspace_view(gm, plan_mid, 512, true, ws + head);
         r = launch_mixed(gm, 512, plan_mid, st);
         if (r != MAGMA_SUCCESS) return r;
     }
@@ -317,7 +316,7 @@ magma_int_t mixed_vbatched(char kind, bool upper, int *n, int *nrhs, double **dA
         MixedSet gs = g;
         gs.index = v.q.ids + offsets[0];
         gs.count = n_small;
-        gs.slot_floats = mixed_factor_floats(max_small);
+        mixed_workspace_view(gs, plan_small, max_small, true, ws + head);
         r = launch_mixed(gs, max_small, plan_small, st);
         if (r != MAGMA_SUCCESS) return r;
     }

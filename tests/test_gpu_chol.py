@@ -10,7 +10,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-SIZES = [1, 2, 3, 7, 8, 15, 16, 17, 31, 32, 33, 47, 64, 65, 100, 128, 129, 256, 300]
+SIZES = [1, 2, 3, 7, 8, 15, 16, 17, 31, 32, 33, 47, 64, 65, 100, 128, 129, 256, 300, 384, 385, 512, 513, 1024]
 
 
 def same(a, b):
@@ -33,7 +33,7 @@ def lower(A):
 @pytest.mark.parametrize("n", SIZES)
 def test_potrf_bitexact(queue, oracle, n):
     import magma_amd as ma
-    count = 200 if n <= 64 else 30
+    count = 200 if n <= 64 else (30 if n <= 300 else (6 if n <= 513 else 3))
     A = spd(count, n, seed=n)
     # poison the strict upper triangle: the routine must neither read nor write it
     Ain = A.copy()
@@ -54,7 +54,7 @@ def test_potrf_bitexact(queue, oracle, n):
         assert oracle.chol_error(A[s], lower(L[s])) < 30
 
 
-@pytest.mark.parametrize("n", [5, 40, 64, 150])
+@pytest.mark.parametrize("n", [5, 40, 64, 150, 385, 1024])
 def test_potrf_info(queue, oracle, n):
     import magma_amd as ma
     count = 12

@@ -789,6 +789,10 @@ def main():
     ap.add_argument("--tune-table", default="",
                     help="load the variant table from this file if it exists (no tuning run), "
                          "else tune and save it there")
+    ap.add_argument("--pin", action="append", default=[], metavar="FAMILY:CLASS:VARIANT",
+                    help="after tuning (or instead of it, with --no-autotune), fix the kernel variant of one "
+                         "size class, e.g. mixed:5:4 (families getrf, mixed, gemm; classes 0-9 as in "
+                         "magma_amd_tune_set); repeatable, for reproducible profiling runs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", dest="check", action="store_false")
     args = ap.parse_args()
@@ -828,6 +832,15 @@ def main():
         tuned = {name: [L.magma_amd_tune_get(fam, c) for c in range(10)]
                  for fam, name in enumerate(("getrf", "mixed", "gemm"))}
         log(f"autotune {time.perf_counter() - t0:.1f} s: {tuned}")
+    families = {"getrf": 0, "mixed": 1, "gemm": 2}
+    for pin in args.pin:
+        fam, cls, var = pin.split(":")
+        if L.magma_amd_tune_set(families[fam], int(cls), int(var)) != 0:
+            raise RuntimeError(f"--pin {pin}: no such family, class or variant")
+    if args.pin:
+        tuned = {name: [L.magma_amd_tune_get(fam, c) for c in range(10)]
+                 for fam, name in enumerate(("getrf", "mixed", "gemm"))}
+        log(f"pinned variants: {tuned}")
     log(f"workload {args.workload}")
     r = WORKLOADS[args.workload](args, torch, ma, L, q)
     roof = r["roofline"]

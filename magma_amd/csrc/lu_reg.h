@@ -198,6 +198,81 @@ __device__ __forceinline__ void row_update_lanes(double (&a)[NMAX], int jj, doub
     }
 }
 
+// dgetrs('N') on the factors still in registers for right-hand sides
+// [c0, c0 + RC) of B (those < nrhs): forward unit-L sweep and backward U
+// sweep, column-oriented as reference dtrsm.
+template <int NMAX, int RC>
+__device__ __forceinline__ void getrs_reg(const double (&a)[NMAX], int pos, int r, int n, double *Bm, int ldb, int c0,
+                                          int nrhs, double *xb)
+{
+    const bool rown = r < n;
+    double y[RC];
+#pragma unroll
+    for (int q = 0; q < RC; ++q)
+        y[q] = (rown && c0 + q < nrhs) ? Bm[r + (size_t)(c0 + q) * ldb] : 0.0;
+#pragma unroll
+    for (int k = 0; k < NMAX; ++k) {
+        if (k < n) {
+            double yk[RC];
+            if constexpr (NMAX == 64) {
+                const unsigned long long ow = __builtin_amdgcn_ballot_w64(pos == k && rown);
+                const int ol = ow ? (int)__builtin_ctzll(ow) : 0;
+#pragma unroll
+                for (int q = 0; q < RC; ++q) yk[q] = readlane_f64(y[q], ol);
+            } else {
+                if (pos == k && rown) {
+#pragma unroll
+                    for (int q = 0; q < RC; ++q) xb[q] = y[q];
+                }
+                wave_lds_sync();
+#pragma unroll
+                for (int q = 0; q < RC; ++q) yk[q] = xb[q];
+            }
+            if (rown && pos > k) {
+#pragma unroll
+                for (int q = 0; q < RC; ++q) y[q] = fma(-yk[q], a[k], y[q]);
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+#pragma unroll
+    for (int k = NMAX - 1; k >= 0; --k) {
+        if (k < n) {
+            double yk[RC];
+            if constexpr (NMAX == 64) {
+                const unsigned long long ow = __builtin_amdgcn_ballot_w64(pos == k && rown);
+                const int ol = ow ? (int)__builtin_ctzll(ow) : 0;
+                if (pos == k && rown) {
+#pragma unroll
+                    for (int q = 0; q < RC; ++q)
+                        if (y[q] != 0.0) y[q] = y[q] / a[k];
+                }
+#pragma unroll
+                for (int q = 0; q < RC; ++q) yk[q] = readlane_f64(y[q], ol);
+            } else {
+                if (pos == k && rown) {
+#pragma unroll
+                    for (int q = 0; q < RC; ++q) {
+                        if (y[q] != 0.0) y[q] = y[q] / a[k];
+                        xb[q] = y[q];
+                    }
+                }
+                wave_lds_sync();
+#pragma unroll
+                for (int q = 0; q < RC; ++q) yk[q] = xb[q];
+            }
+            if (rown && pos < k) {
+#pragma unroll
+                for (int q = 0; q < RC; ++q) y[q] = fma(-yk[q], a[k], y[q]);
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < RC; ++q)
+        if (rown && c0 + q < nrhs) Bm[pos + (size_t)(c0 + q) * ldb] = y[q];
+}
+
 // ---------------------------------------------------------------------------
 // LU, partial pivoting.
 template <int NMAX, bool SOLVE, bool VAR>
@@ -317,74 +392,11 @@ __attribute__((amdgpu_waves_per_eu(reg_waves_per_eu<NMAX>()))) getrf_reg_kernel(
             const int nrhs = s.nrhs_arr ? s.nrhs_arr[it.id] : s.nrhs;
             const int ldb = s.ldb_arr ? s.ldb_arr[it.id] : s.ldb;
             double *Bm = s.B[it.id];
-            const bool rown = r < n;
-            for (int c0 = 0; c0 < nrhs; c0 += kRhsChunk) {
-                double y[kRhsChunk];
-    #pragma unroll
-                for (int q = 0; q < kRhsChunk; ++q)
-                    y[q] = (rown && c0 + q < nrhs) ? Bm[r + (size_t)(c0 + q) * ldb] : 0.0;
-    #pragma unroll
-                for (int k = 0; k < NMAX; ++k) {
-                    if (k < n) {
-                        double yk[kRhsChunk];
-                        if constexpr (NMAX == 64) {
-                            const unsigned long long ow = __builtin_amdgcn_ballot_w64(pos == k && rown);
-                            const int ol = ow ? (int)__builtin_ctzll(ow) : 0;
-    #pragma unroll
-                            for (int q = 0; q < kRhsChunk; ++q) yk[q] = readlane_f64(y[q], ol);
-                        } else {
-                            if (pos == k && rown) {
-    #pragma unroll
-                                for (int q = 0; q < kRhsChunk; ++q) xb[q] = y[q];
-                            }
-                            wave_lds_sync();
-    #pragma unroll
-                            for (int q = 0; q < kRhsChunk; ++q) yk[q] = xb[q];
-                        }
-                        if (rown && pos > k) {
-    #pragma unroll
-                            for (int q = 0; q < kRhsChunk; ++q) y[q] = fma(-yk[q], a[k], y[q]);
-                        }
-                        __builtin_amdgcn_wave_barrier();
-                    }
-                }
-    #pragma unroll
-                for (int k = NMAX - 1; k >= 0; --k) {
-                    if (k < n) {
-                        double yk[kRhsChunk];
-                        if constexpr (NMAX == 64) {
-                            const unsigned long long ow = __builtin_amdgcn_ballot_w64(pos == k && rown);
-                            const int ol = ow ? (int)__builtin_ctzll(ow) : 0;
-                            if (pos == k && rown) {
-    #pragma unroll
-                                for (int q = 0; q < kRhsChunk; ++q)
-                                    if (y[q] != 0.0) y[q] = y[q] / a[k];
-                            }
-    #pragma unroll
-                            for (int q = 0; q < kRhsChunk; ++q) yk[q] = readlane_f64(y[q], ol);
-                        } else {
-                            if (pos == k && rown) {
-    #pragma unroll
-                                for (int q = 0; q < kRhsChunk; ++q) {
-                                    if (y[q] != 0.0) y[q] = y[q] / a[k];
-                                    xb[q] = y[q];
-                                }
-                            }
-                            wave_lds_sync();
-    #pragma unroll
-                            for (int q = 0; q < kRhsChunk; ++q) yk[q] = xb[q];
-                        }
-                        if (rown && pos < k) {
-    #pragma unroll
-                            for (int q = 0; q < kRhsChunk; ++q) y[q] = fma(-yk[q], a[k], y[q]);
-                        }
-                        __builtin_amdgcn_wave_barrier();
-                    }
-                }
-    #pragma unroll
-                for (int q = 0; q < kRhsChunk; ++q)
-                    if (rown && c0 + q < nrhs) Bm[pos + (size_t)(c0 + q) * ldb] = y[q];
-            }
+            // one right-hand side at a time when the system has one (a group
+            // of four otherwise would carry three columns of zeros)
+            if (nrhs == 1) getrs_reg<NMAX, 1>(a, pos, r, n, Bm, ldb, 0, 1, xb);
+            else
+                for (int c0 = 0; c0 < nrhs; c0 += kRhsChunk) getrs_reg<NMAX, kRhsChunk>(a, pos, r, n, Bm, ldb, c0, nrhs, xb);
         }
     }
 }
@@ -400,6 +412,9 @@ __attribute__((amdgpu_waves_per_eu(reg_waves_per_eu<NMAX>()))) potrf_reg_kernel(
     constexpr int CH = reg_chunk<NMAX>();
     __shared__ double colbuf[kWavesPerBlock][kWave];
     __shared__ double aux[kWavesPerBlock][MPW][kRhsChunk];
+    // NMAX = 64: rows of L handed to the L' sweep, kLRows at a time
+    constexpr int kLRows = 8;
+    __shared__ double lrows[kWavesPerBlock][NMAX == 64 ? kLRows : 1][kWave];
 
     const int lane = threadIdx.x & (kWave - 1);
     const int w = threadIdx.x / kWave;
@@ -431,6 +446,7 @@ __attribute__((amdgpu_waves_per_eu(reg_waves_per_eu<NMAX>()))) potrf_reg_kernel(
         buf[r] = 0.0;
 
         int info = 0;
+        double dg = 1.0;                         // this lane's diagonal entry l_rr
     #pragma unroll 1
         for (int jbase = 0; jbase < NMAX; jbase += CH) {
             const int kmax = NMAX - jbase;
@@ -453,7 +469,10 @@ __attribute__((amdgpu_waves_per_eu(reg_waves_per_eu<NMAX>()))) potrf_reg_kernel(
                         const double rc = 1.0 / ljj;
                         const bool below = rowok && r > j;
                         const double l = rc * a[jj];
-                        if (r == j) a[jj] = ljj;
+                        if (r == j) {
+                            a[jj] = ljj;
+                            dg = ljj;
+                        }
                         if constexpr (NMAX == 64) {
                             // trailing update a_rc -= l_rj * l_cj (dsyrk), c in (j, n):
                             // l_cj is lane c's multiplier (jbase is 0: all columns unrolled)
@@ -530,6 +549,36 @@ __attribute__((amdgpu_waves_per_eu(reg_waves_per_eu<NMAX>()))) potrf_reg_kernel(
                         __builtin_amdgcn_wave_barrier();
                     }
                 }
+                if constexpr (NMAX == 64) {
+                    // the same sweep, rolled: rows of L go to LDS kLRows at a
+                    // time (every lane of the group writes its own row at once)
+                    // and x_k reaches the other lanes by readlane
+    #pragma unroll 1
+                    for (int kb = ((n - 1) / kLRows) * kLRows; kb >= 0; kb -= kLRows) {
+                        if (rowok && r >= kb && r < kb + kLRows) {
+    #pragma unroll
+                            for (int c = 0; c < NMAX; ++c)
+                                if (c < r) lrows[w][r - kb][c] = a[c];
+                        }
+                        wave_lds_sync();
+    #pragma unroll 1
+                        for (int k = min(kb + kLRows - 1, n - 1); k >= kb; --k) {
+                            if (r == k) {
+    #pragma unroll
+                                for (int q = 0; q < kRhsChunk; ++q) y[q] = y[q] / dg;
+                            }
+                            double yk[kRhsChunk];
+    #pragma unroll
+                            for (int q = 0; q < kRhsChunk; ++q) yk[q] = readlane_f64(y[q], k);
+                            if (rowok && r < k) {
+                                const double lk = lrows[w][k - kb][r];
+    #pragma unroll
+                                for (int q = 0; q < kRhsChunk; ++q) y[q] = fma(-lk, yk[q], y[q]);
+                            }
+                        }
+                        wave_lds_sync();
+                    }
+                } else {
     #pragma unroll
                 for (int k = NMAX - 1; k >= 0; --k) {
                     if (k < n) {
@@ -550,6 +599,7 @@ __attribute__((amdgpu_waves_per_eu(reg_waves_per_eu<NMAX>()))) potrf_reg_kernel(
                         }
                         __builtin_amdgcn_wave_barrier();
                     }
+                }
                 }
     #pragma unroll
                 for (int q = 0; q < kRhsChunk; ++q)

@@ -3,7 +3,9 @@
 //
 // Each wavefront owns one 16x16, 32x32 or 48x48 tile of C (1, 4 or 9 MFMA
 // blocks of 16x16; the variant is tuned per size class) and
-// walks k four at a time; a workgroup holds 4 independent wavefronts.  The
+// walks k four at a time; a workgroup holds 4 independent wavefronts.  A
+// fourth variant gives a workgroup one 64x64 tile with its operands staged
+// through LDS (gemm_lds_kernel).  The
 // grid is persistent: wavefronts stride over a flat tile list.  For vbatched
 // calls the list is the device prefix sum of per-matrix tile counts
 // (workqueue.hip), so a matrix contributes exactly its own tiles instead of a
@@ -159,6 +161,155 @@ __global__ void __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_e
     }
 }
 
+// Variant with a 64 x 64 C tile per workgroup.  The k blocks of A and B pass
+// through LDS (double-buffered, one barrier per 16 k), so a workgroup reads
+// each operand once instead of once per wavefront; the four wavefronts own
+// 32 x 32 quadrants.  Every entry is accumulated in the same k order as the
+// per-wavefront tiles (16-k steps with the in-step permutation
+// k0 + 4 lk + sl, then the tail four at a time), so the variants agree bit for
+// bit.
+constexpr int kLT = 64;           // workgroup tile edge
+constexpr int kLK = 16;           // k per stage
+constexpr int kLPitch = kLT + 4;  // LDS row pitch, in doubles
+
+template <bool TA, bool TB>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+gemm_lds_kernel(GemmSet g, const int *prefix, int fixed_tiles)
+{
+    __shared__ double As[2][kLK][kLPitch];   // [stage][k][row within the tile]
+    __shared__ double Bs[2][kLK][kLPitch];   // [stage][k][column within the tile]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int wr = wave & 1, wc = wave >> 1;  // this wavefront's quadrant
+    // element of a stage each thread moves: A (row ar + 16 j or + 4 j, k ak),
+    // B (k bk, column bc), chosen so the global loads are unit-stride
+    const int ar = TA ? tid / kLK : tid % kLT, ak = TA ? tid % kLK : tid / kLT;
+    const int bc = TB ? tid % kLT : tid / kLK, bk = TB ? tid / kLT : tid % kLK;
+    constexpr int AST = TA ? 256 / kLK : 0, AKT = TA ? 0 : 256 / kLT;   // strides over j
+    constexpr int BST = TB ? 0 : 256 / kLK, BKT = TB ? 256 / kLT : 0;
+    const int total = prefix ? prefix[g.count] : fixed_tiles * g.count;
+
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+        int s, local;
+        if (prefix) {
+            int lo = 0, hi = g.count;          // largest s with prefix[s] <= t
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (prefix[mid] <= t) lo = mid; else hi = mid;
+            }
+            s = lo;
+            local = t - prefix[s];
+        } else {
+            s = t / fixed_tiles;
+            local = t - s * fixed_tiles;
+        }
+        const int id = g.index ? g.index[s] : s;
+        const int m = pick(g.m_arr, g.m, id), n = pick(g.n_arr, g.n, id), k = pick(g.k_arr, g.k, id);
+        const int lda = pick(g.lda_arr, g.lda, id), ldb = pick(g.ldb_arr, g.ldb, id);
+        const int ldc = pick(g.ldc_arr, g.ldc, id);
+        const int mt = (m + kLT - 1) / kLT;
+        const int row0 = (local % mt) * kLT;
+        const int col0 = (local / mt) * kLT;
+        const double *A = g.A[id];
+        const double *B = g.B[id];
+        double *C = g.C[id];
+
+        d4 acc[2][2];
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = d4{0.0, 0.0, 0.0, 0.0};
+
+        const int kmain = k & ~(kLK - 1);
+        double ra[4], rb[4];
+        auto gload = [&](int kb) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int r = row0 + ar + AST * j, kk = kb + ak + AKT * j;
+                const int c = col0 + bc + BST * j, kq = kb + bk + BKT * j;
+                ra[j] = (r < m) ? (TA ? A[kk + (size_t)r * lda] : A[r + (size_t)kk * lda]) : 0.0;
+                rb[j] = (c < n) ? (TB ? B[c + (size_t)kq * ldb] : B[kq + (size_t)c * ldb]) : 0.0;
+            }
+        };
+        auto sstore = [&](int st) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                As[st][ak + AKT * j][ar + AST * j] = ra[j];
+                Bs[st][bk + BKT * j][bc + BST * j] = rb[j];
+            }
+        };
+        if (kmain > 0) {
+            gload(0);
+            sstore(0);
+        }
+        __syncthreads();
+        int st = 0;
+#pragma unroll 1
+        for (int kb = 0; kb < kmain; kb += kLK) {
+            const bool more = kb + kLK < kmain;
+            if (more) gload(kb + kLK);
+#pragma unroll
+            for (int sl = 0; sl < 4; ++sl) {
+                const int kk = 4 * lk + sl;
+                double a[2], b[2];
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    a[q] = As[st][kk][32 * wr + 16 * q + li];
+                    b[q] = Bs[st][kk][32 * wc + 16 * q + li];
+                }
+#pragma unroll
+                for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                    for (int bj = 0; bj < 2; ++bj)
+                        acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[bi], b[bj], acc[bi][bj], 0, 0, 0);
+            }
+            if (more) sstore(st ^ 1);
+            __syncthreads();
+            st ^= 1;
+        }
+        // tail, 4 k per step in order (as the per-wavefront tiles)
+        for (int k0 = kmain; k0 < k; k0 += 4) {
+            const int kk = k0 + lk;
+            const bool kin = kk < k;
+            double a[2], b[2];
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const int r = row0 + 32 * wr + 16 * q + li, c = col0 + 32 * wc + 16 * q + li;
+                a[q] = 0.0;
+                b[q] = 0.0;
+                if (kin) {
+                    if (r < m) a[q] = TA ? A[kk + (size_t)r * lda] : A[r + (size_t)kk * lda];
+                    if (c < n) b[q] = TB ? B[c + (size_t)kk * ldb] : B[kk + (size_t)c * ldb];
+                }
+            }
+#pragma unroll
+            for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < 2; ++bj)
+                    acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[bi], b[bj], acc[bi][bj], 0, 0, 0);
+        }
+        const double alpha = g.alpha, beta = g.beta;
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi) {
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj) {
+                const int col = col0 + 32 * wc + 16 * bj + li;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int row = row0 + 32 * wr + 16 * bi + lk + 4 * i;
+                    if (row < m && col < n) {
+                        double *c = C + row + (size_t)col * ldc;
+                        // beta == 0: C is not read (BLAS semantics)
+                        *c = (beta == 0.0) ? alpha * acc[bi][bj][i] : fma(alpha, acc[bi][bj][i], beta * *c);
+                    }
+                }
+            }
+        }
+        // the next tile's first stage overwrites stage 0
+        __syncthreads();
+    }
+}
+
 // C = beta C when k == 0 or alpha == 0 (BLAS quick path)
 __global__ void gemm_scale_kernel(GemmSet g, const int *prefix, int fixed_tiles, int ts)
 {
@@ -205,18 +356,24 @@ static void launch_bt(const GemmSet &g, const int *prefix, int fixed_tiles, int 
 }
 
 // prefix: device tile prefix (vbatched) or null (fixed sizes), built for tile
-// edge `tile` (16, 32 or 48).  max_tiles bounds the flat tile count for the grid size.
+// edge `tile` (16, 32, 48: one tile per wavefront; 64: one per workgroup).
+// max_tiles bounds the flat tile count for the grid size.
 magma_int_t launch_gemm_tiles(const GemmSet &g, const int *prefix, long max_tiles, int tile, hipStream_t st)
 {
     if (g.count <= 0 || max_tiles <= 0) return MAGMA_SUCCESS;
-    if (tile != 16 && tile != 32 && tile != 48) return MAGMA_ERR_NOT_SUPPORTED;
+    if (tile != 16 && tile != 32 && tile != 48 && tile != kLT) return MAGMA_ERR_NOT_SUPPORTED;
     const int fixed_tiles = prefix ? 0 : ceil_div(g.m, tile) * ceil_div(g.n, tile);
     // enough resident wavefronts to fill 256 CUs several times over
-    const long want = (max_tiles + kWaves - 1) / kWaves;
+    const long want = tile == kLT ? max_tiles : (max_tiles + kWaves - 1) / kWaves;
     const int grid = (int)std::min<long>(want, 256L * 16);
     const bool scale_only = (!prefix && (g.k == 0 || g.alpha == 0.0)) || g.alpha == 0.0;
     if (scale_only) {
         gemm_scale_kernel<<<grid, 256, 0, st>>>(g, prefix, fixed_tiles, tile);
+    } else if (tile == kLT) {
+        if (g.ta == 'N' && g.tb == 'N') gemm_lds_kernel<false, false><<<grid, 256, 0, st>>>(g, prefix, fixed_tiles);
+        else if (g.ta == 'N') gemm_lds_kernel<false, true><<<grid, 256, 0, st>>>(g, prefix, fixed_tiles);
+        else if (g.tb == 'N') gemm_lds_kernel<true, false><<<grid, 256, 0, st>>>(g, prefix, fixed_tiles);
+        else gemm_lds_kernel<true, true><<<grid, 256, 0, st>>>(g, prefix, fixed_tiles);
     } else if (tile == 48) {
         launch_bt<3>(g, prefix, fixed_tiles, grid, st);
     } else if (tile == 32) {

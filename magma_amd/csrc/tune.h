@@ -13,7 +13,8 @@ enum TuneFamily {
                        //   per CU, 2 VALU blocked kernel
     kTuneMixed = 1,    // mixed-precision solver: block-column width / right-hand-side group variants,
                        //   and workgroups per CU
-    kTuneGemm = 2,     // batched / vbatched DGEMM: 0 32x32 tiles per wavefront, 1 16x16, 2 48x48
+    kTuneGemm = 2,     // batched / vbatched DGEMM: 0 32x32 tiles per wavefront, 1 16x16, 2 48x48,
+                       //   3 64x64 per workgroup through LDS
     kTuneFamilies = 3
 };
 constexpr int kTuneClasses = 10;     // size classes of workqueue.h
@@ -21,7 +22,7 @@ constexpr int kTuneClasses = 10;     // size classes of workqueue.h
 // Variant for (family, size class), the default when untuned or out of range.
 int tuned_variant(TuneFamily f, int size_class);
 // C tile edge of a DGEMM variant
-inline int gemm_tile_of(int variant) { return variant == 1 ? 16 : (variant == 2 ? 48 : 32); }
+inline int gemm_tile_of(int variant) { return variant == 1 ? 16 : (variant == 2 ? 48 : (variant == 3 ? 64 : 32)); }
 int size_class_of(int dim);
 // number of variants of a family
 int tune_variant_count(TuneFamily f);

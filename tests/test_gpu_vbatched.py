@@ -229,8 +229,9 @@ def test_vbatched_argument_errors(queue):
 
 def test_gemm_tile_variants_bitwise_equal(queue, oracle):
     """Every C entry is accumulated over k in the same order whatever the tile
-    edge (16, 32, 48), so the three GEMM variants agree bit for bit; each also
-    meets the tester's bound and is exact on integer operands."""
+    (16, 32, 48 per wavefront, 64 per workgroup through LDS), so the four GEMM
+    variants agree bit for bit; each also meets the tester's bound and is exact
+    on integer operands in all four transpose cases."""
     import magma_amd as ma
     from magma_amd import gen
     L = ma.load()
@@ -239,12 +240,19 @@ def test_gemm_tile_variants_bitwise_equal(queue, oracle):
     Bs = [gen.uniform_pm1(103, [i], n * n).reshape(n, n) for i, n in enumerate(sizes)]
     Cs = [gen.uniform_pm1(203, [i], n * n).reshape(n, n) for i, n in enumerate(sizes)]
     rng = np.random.default_rng(9)
-    Ai = rng.integers(-4, 5, (30, 50, 61)).astype(float)     # m = 61, k = 50
-    Bi = rng.integers(-4, 5, (30, 53, 50)).astype(float)     # n = 53
-    Ci = rng.integers(-4, 5, (30, 53, 61)).astype(float)
+    m, n, k = 61, 131, 50                                    # k: three 16-steps and a 2-wide tail
+    trans = {}
+    for ta in "NT":
+        for tb in "NT":
+            sa = (k, m) if ta == "N" else (m, k)             # storage (cols, rows)
+            sb = (n, k) if tb == "N" else (k, n)
+            trans[ta + tb] = (rng.integers(-4, 5, (30,) + sa).astype(float),
+                              rng.integers(-4, 5, (30,) + sb).astype(float),
+                              rng.integers(-4, 5, (30, n, m)).astype(float))
+    t = {"N": ma.MagmaNoTrans, "T": ma.MagmaTrans}
     outs = []
     try:
-        for v in range(3):
+        for v in range(4):
             for c in range(10):
                 assert L.magma_amd_tune_set(2, c, v) == 0
             dA, dB, dC = ma.VarBatch(As), ma.VarBatch(Bs), ma.VarBatch(Cs)
@@ -258,17 +266,19 @@ def test_gemm_tile_variants_bitwise_equal(queue, oracle):
             worst = max(_gemm_ref_error(As[i], Bs[i], Cs[i], got[i], 1.0, 1.0, oracle, n, n, n)
                         for i, n in enumerate(sizes))
             assert worst < 3 * EPS, (v, worst)
-            dAi, dBi, dCi = ma.DeviceBatch(Ai), ma.DeviceBatch(Bi), ma.DeviceBatch(Ci)
-            L.magma_dgemm_batched(ma.MagmaNoTrans, ma.MagmaNoTrans, 61, 53, 50, 2.0, dAi.ptr, dAi.ld, dBi.ptr,
-                                  dBi.ld, -1.0, dCi.ptr, dCi.ld, 30, queue.handle)
-            queue.sync()
-            gi = dCi.numpy()
-            for s in range(30):
-                ref = oracle.dgemm(b"N", b"N", 61, 53, 50, 2.0, Ai[s], 61, Bi[s], 50, -1.0, Ci[s], 61)
-                assert np.array_equal(gi[s], ref), (v, s)
+            for key, (Ai, Bi, Ci) in trans.items():
+                dAi, dBi, dCi = ma.DeviceBatch(Ai), ma.DeviceBatch(Bi), ma.DeviceBatch(Ci)
+                L.magma_dgemm_batched(t[key[0]], t[key[1]], m, n, k, 2.0, dAi.ptr, dAi.ld, dBi.ptr,
+                                      dBi.ld, -1.0, dCi.ptr, dCi.ld, 30, queue.handle)
+                queue.sync()
+                gi = dCi.numpy()
+                for s in range(30):
+                    ref = oracle.dgemm(key[0].encode(), key[1].encode(), m, n, k, 2.0, Ai[s], dAi.ld, Bi[s],
+                                       dBi.ld, -1.0, Ci[s], m)
+                    assert np.array_equal(gi[s], ref), (v, key, s)
     finally:
         for c in range(10):
             L.magma_amd_tune_set(2, c, -1)
-    for v in (1, 2):
+    for v in (1, 2, 3):
         for i in range(len(sizes)):
             assert np.array_equal(outs[v][i], outs[0][i]), (v, sizes[i])

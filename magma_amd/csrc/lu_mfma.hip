@@ -472,15 +472,20 @@ static int resident_per_cu(K kernel)
 }
 
 magma_int_t launch_getrf_blocked(const MatSet &s, int max_m, hipStream_t st);
+magma_int_t launch_getrf_reg128(const MatSet &s, int max_dim, int waves_per_simd, hipStream_t st);
 
 // 64 < max(m, n) <= 512; returns MAGMA_ERR_NOT_SUPPORTED above.  Variant
 // (tuned per size class): 0 as many workgroups per CU as fit, 1 one workgroup
-// per CU, 2 the VALU blocked kernel.
+// per CU, 2 the VALU blocked kernel, 3 / 4 the register-resident kernel
+// (lu_reg128.hip) at three / two wavefronts per SIMD, for max(m, n) <= 128
+// (variant 0 above).
 magma_int_t launch_getrf_mfma(const MatSet &s, int max_dim, hipStream_t st)
 {
     if (s.count <= 0) return MAGMA_SUCCESS;
     if (max_dim > 512) return MAGMA_ERR_NOT_SUPPORTED;
-    const int v = tuned_variant(kTuneGetrf, size_class_of(max_dim));
+    int v = tuned_variant(kTuneGetrf, size_class_of(max_dim));
+    if (v >= 3 && max_dim <= 128) return launch_getrf_reg128(s, max_dim, v == 3 ? 3 : 2, st);
+    if (v >= 3) v = 0;
     if (v == 2) return launch_getrf_blocked(s, max_dim, st);
     if (max_dim <= 256) {
         const int grid = persistent_grid(s.count, v == 1 ? 1 : resident_per_cu(getrf_mfma_kernel<256>));

@@ -10,7 +10,8 @@ namespace magma_amd {
 
 enum TuneFamily {
     kTuneGetrf = 0,    // LU, 64 < n <= 512: 0 MFMA kernel at full occupancy, 1 MFMA one workgroup
-                       //   per CU, 2 VALU blocked kernel
+                       //   per CU, 2 VALU blocked kernel, 3 / 4 register-resident kernel at 3 / 2 waves per
+                       //   SIMD (n <= 128 only)
     kTuneMixed = 1,    // mixed-precision solver: block-column width / right-hand-side group variants,
                        //   and workgroups per CU
     kTuneGemm = 2,     // batched / vbatched DGEMM: 0 32x32 tiles per wavefront, 1 16x16, 2 48x48,

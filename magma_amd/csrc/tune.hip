@@ -32,7 +32,10 @@ void init_table()
     g_init = true;
 }
 
-const int kVariants[kTuneFamilies] = {3, 5, 4};
+const int kVariants[kTuneFamilies] = {5, 5, 4};
+
+// variants a size class can use: the register-resident LU (3, 4) only up to order 128
+int class_variants(int f, int c) { return (f == kTuneGetrf && c > 4) ? 3 : kVariants[f]; }
 
 __global__ void fill_kernel(double *p, size_t count, unsigned seed)
 {
@@ -128,7 +131,7 @@ int tuned_variant(TuneFamily f, int c)
 {
     if (!g_init || f < 0 || f >= kTuneFamilies || c < 0 || c >= kTuneClasses) return 0;
     const int v = g_table[f][c];
-    return (v < 0 || v >= kVariants[f]) ? 0 : v;
+    return (v < 0 || v >= class_variants(f, c)) ? 0 : v;
 }
 
 }  // namespace magma_amd
@@ -142,6 +145,7 @@ magma_int_t tune_getrf(hipStream_t st, FILE *log)
     const int classes[] = {4, 5, 6};
     const int dims[] = {96, 192, 384};
     const int counts[] = {4096, 2048, 512};
+    const int nv[] = {5, 3, 3};            // the register-resident variants cover n <= 128 only
     for (int t = 0; t < 3; ++t) {
         const int n = dims[t], count = counts[t];
         const size_t elems = (size_t)n * n * count;
@@ -158,7 +162,7 @@ magma_int_t tune_getrf(hipStream_t st, FILE *log)
         s.count = count;
         float best = 1e30f;
         int bestv = 0;
-        for (int v = 0; v < kVariants[kTuneGetrf]; ++v) {
+        for (int v = 0; v < nv[t]; ++v) {
             g_table[kTuneGetrf][classes[t]] = v;
             const float ms = best_ms(
                 st, 3, [&]() { (void)hipMemcpyAsync(work.p, src.p, elems * 8, hipMemcpyDeviceToDevice, st); },
@@ -327,7 +331,7 @@ magma_int_t magma_amd_tune_set(magma_int_t family, magma_int_t size_class, magma
 {
     if (family < 0 || family >= kTuneFamilies) return -1;
     if (size_class < 0 || size_class >= kTuneClasses) return -2;
-    if (variant < -1 || variant >= kVariants[family]) return -3;
+    if (variant < -1 || variant >= class_variants(family, size_class)) return -3;
     init_table();
     g_table[family][size_class] = variant;
     return MAGMA_SUCCESS;

@@ -5,7 +5,10 @@ autotuned table itself on the benchmark's conditioned n = 256 inputs.
   LU (family 0, classes 4-6: orders up to 128, 256, 512): pivots, info and
   factors bit-exact against the oracle's netlib-order dgetrf, on uniform
   matrices and on the conditioned U diag(s) V' matrices the mixed benchmark
-  uses.
+  uses.  The register-resident variants of class 4 (lu_reg128.hip) also on
+  rectangular and padded shapes, through the vbatched entry point, and on
+  integer matrices with zero columns, exact ties, NaN and Inf entries (the
+  last two take its exact path part-way through the factorization).
   Mixed solver (family 1, classes 4-6): dsgesv and dsposv backward error
   ||b - Ax||_inf / (||A||_inf ||x||_inf) <= n eps64 for every converged matrix
   (orc_solve_error <= 1), the stopping test of src/zcgesv_gpu.cpp:266-279;
@@ -52,7 +55,7 @@ def _same(a, b):
 
 
 LU_CASES = [(c, v, n) for c, ns in ((4, (65, 100, 128)), (5, (129, 200, 256)), (6, (257, 300, 512)))
-            for v in range(3) for n in ns]
+            for v in range(5 if c == 4 else 3) for n in ns]
 
 
 @pytest.mark.parametrize("cls,variant,n", LU_CASES)
@@ -72,6 +75,92 @@ def test_getrf_variant_bitexact(queue, oracle, table, cls, variant, n):
     assert np.array_equal(info.cpu().numpy(), info_o)
     assert np.array_equal(ipiv.numpy(), ipo)
     assert _same(dA.numpy(), LUo)
+
+
+REG128_SHAPES = [(65, 65, None), (128, 128, None), (128, 100, None), (100, 128, None), (128, 9, None),
+                 (9, 128, None), (70, 127, 75), (127, 70, 130), (128, 128, 131)]
+
+
+@pytest.mark.parametrize("variant", [3, 4])
+@pytest.mark.parametrize("m,n,ld", REG128_SHAPES)
+def test_getrf_reg128_shapes(queue, oracle, table, variant, m, n, ld):
+    import magma_amd as ma
+    from magma_amd import gen
+    assert table.magma_amd_tune_set(GETRF, 4, variant) == 0
+    count = 20
+    A = gen.uniform_pm1(900 + m + n, np.arange(count), m * n).reshape(count, n, m).copy()
+    print(f"getrf reg128 v={variant} m={m} n={n} ld={ld}", flush=True)
+    dA = ma.DeviceBatch(A, ld=ld)
+    ret, ipiv, info = ma.dgetrf_batched(dA, queue, m=m, n=n)
+    assert ret == 0
+    got, ip, inf = dA.numpy(), ipiv.numpy(), info.cpu().numpy()
+    for s in range(count):
+        LUo, ipo, info_o = oracle.dgetrf(A[s], m=m, n=n)
+        assert inf[s] == info_o, s
+        assert np.array_equal(ip[s], ipo), s
+        assert _same(got[s], LUo), s
+
+
+@pytest.mark.parametrize("variant", [3, 4])
+def test_getrf_reg128_special(queue, oracle, table, variant):
+    """Integer entries (exact ties in the pivot search), zero columns (info,
+    the factorization continues), duplicated rows, NaN and Inf entries at
+    different depths, an all-zero and a permutation matrix."""
+    import magma_amd as ma
+    assert table.magma_amd_tune_set(GETRF, 4, variant) == 0
+    n = 97
+    rng = np.random.default_rng(97)
+    mats = []
+    for t in range(30):
+        A = rng.integers(-2, 3, size=(n, n)).astype(float)
+        if t % 6 == 1:
+            A[:, (5 * t) % n] = 0.0
+        if t % 6 == 2:
+            A[(t + 1) % n, :] = A[t % n, :]
+        if t % 6 == 3:
+            A[(t * 7) % n, (t * 3) % n] = np.nan
+        if t % 6 == 4:
+            A[(t * 11) % n, (t * 5) % n] = np.inf
+            A[(t * 13) % n, (t * 5 + 40) % n] = -np.inf
+        if t % 6 == 5:
+            A = np.abs(A) * 1e300                     # products overflow to Inf during the update
+        mats.append(A.T.copy())
+    mats.append(np.zeros((n, n)))
+    mats.append(np.eye(n)[::-1].T.copy())
+    A = np.stack(mats)
+    print(f"getrf reg128 special v={variant}", flush=True)
+    dA = ma.DeviceBatch(A)
+    ret, ipiv, info = ma.dgetrf_batched(dA, queue)
+    assert ret == 0
+    LUo, ipo, info_o = oracle.batch_dgetrf(A)
+    assert np.array_equal(info.cpu().numpy(), info_o)
+    assert np.array_equal(ipiv.numpy(), ipo)
+    assert _same(dA.numpy(), LUo)
+
+
+@pytest.mark.parametrize("variant", [3, 4])
+def test_getrf_reg128_vbatched(queue, oracle, table, variant):
+    import magma_amd as ma
+    from magma_amd import gen
+    assert table.magma_amd_tune_set(GETRF, 4, variant) == 0
+    rng = np.random.default_rng(128 + variant)
+    shapes = [(int(rng.integers(1, 129)), int(rng.integers(65, 129))) for _ in range(40)]
+    shapes = [(m, n) if i % 2 else (n, m) for i, (m, n) in enumerate(shapes)]
+    mats = [gen.uniform_pm1(77, [i], m * n).reshape(n, m).copy() for i, (m, n) in enumerate(shapes)]
+    print(f"getrf reg128 vbatched v={variant}", flush=True)
+    V = ma.VarBatch(mats)
+    ipiv = ma.VarInts([min(m, n) for m, n in shapes])
+    info = ma.dev_ints(np.zeros(len(mats)))
+    ret = table.magma_dgetrf_vbatched(V.d_rows.data_ptr(), V.d_cols.data_ptr(), V.ptr, V.d_ld.data_ptr(),
+                                      ipiv.ptr, info.data_ptr(), V.count, queue.handle)
+    assert ret == 0
+    got = V.matrices()
+    inf = info.cpu().numpy()
+    for i, (m, n) in enumerate(shapes):
+        LUo, ipo, info_o = oracle.dgetrf(mats[i], m=m, n=n)
+        assert inf[i] == info_o, i
+        assert np.array_equal(ipiv.numpy(i, min(m, n)), ipo), i
+        assert _same(got[i], LUo), i
 
 
 @pytest.mark.parametrize("variant", range(3))

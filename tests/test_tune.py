@@ -26,7 +26,9 @@ def test_defaults_are_untuned(L):
 def test_set_rejects_out_of_range(L):
     assert L.magma_amd_tune_set(3, 0, 0) == -1           # no such family
     assert L.magma_amd_tune_set(0, 10, 0) == -2          # no such size class
-    assert L.magma_amd_tune_set(0, 5, 3) == -3           # LU has variants 0..2
+    assert L.magma_amd_tune_set(0, 5, 3) == -3           # LU above order 128 has variants 0..2
+    assert L.magma_amd_tune_set(0, 4, 5) == -3           # up to order 128, 0..4
+    assert L.magma_amd_tune_set(0, 4, 4) == 0
     assert L.magma_amd_tune_set(2, 5, 4) == -3           # GEMM has variants 0..3
     assert L.magma_amd_tune_get(0, 5) == -1
     assert L.magma_amd_tune_get(7, 0) == -1

@@ -12,13 +12,14 @@
 //          thread picks the winning wavefront's row as the pivot row.  Row
 //          interchanges inside the panel are logical (pos); rows are written
 //          back to their final positions once, at the end of the panel.
-//   rows   One thread per column right of the panel: the panel's
-//          interchanges as a gather (all loads, then all stores, so nothing is
-//          read after it is overwritten), and the unit-lower solve
-//          U12 = L11^-1 A12 in registers, L11 from LDS.  -U12 is also staged
-//          in LDS for the next phase.  The columns left of the panel (the L
-//          part) take all later interchanges in one pass after the last panel,
-//          a column read and written once, instead of a gather per panel.
+//   rows   The panel's interchanges as a gather of the 32-row block right of
+//          the panel into LDS, 32 lanes per column (coalesced), the old top
+//          rows stored to the rows they move to; then one thread per column
+//          solves U12 = L11^-1 A12 in registers (L11 from LDS) and leaves -U12
+//          in LDS for the update; then U12 is stored back, 32 lanes per
+//          column.  The columns left of the panel (the L part) take all later
+//          interchanges in one pass after the last panel, a column read and
+//          written once, instead of a gather per panel.
 //   update A22 -= L21 U12 in 32x32 tiles, one tile per wavefront at a time, on
 //          v_mfma_f64_16x16x4f64.  The product is formed transposed
 //          (A22^T -= U12^T L21^T) so the C tile and the L21 operand are read
@@ -108,7 +109,8 @@ template <int MT>
 struct LuSmem {
     PivSlot slot[2][kWaves];
     double L11[PW][PW + 1];
-    double Us[PW][MT];        // -U12 of the current panel, [k][column - (j0 + jb)]
+    double Us[PW][MT + 1];    // -U12 of the current panel, [k][column - (j0 + jb)]; the odd
+                              // pitch spreads a column's entries over the banks
     int src_top[PW];          // logical panel row r came from physical panel row src_top[r]
     int down_dst[PW];         // physical panel row r (< jb) moves to logical row down_dst[r] (>= jb), or -1
     int swaps[PW];
@@ -283,56 +285,87 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(M
             const int cb = j0 + jb;                 // first column right of the panel
             // Columns right of the panel only: the columns left of it (the L
             // part) get all later interchanges in one pass after the last
-            // panel.  Logical top row r comes from panel row st[r]; indices
-            // past jb are clamped to valid rows (their values are discarded),
-            // so every load is unconditional and all are in flight together.
+            // panel.  Three passes over the jb x nright block U12:
+            //   gather  32 consecutive lanes per column (coalesced 256-byte
+            //           reads): logical top row r from panel row src_top[r]
+            //           into Us[r][column], the old top row r to down_dst[r];
+            //           a column's reads all precede its writes (one
+            //           half-wavefront owns it);
+            //   solve   thread per column, U12 = L11^-1 A12 on Us in place
+            //           (stored negated, the update's operand);
+            //   store   32 lanes per column again, U12 back to A.
             const int nright = n - cb;
+            constexpr int kCPI = kThreads / PW;     // columns per pass of the workgroup
+            constexpr int kBatch = 8;               // passes whose loads are in flight together
+            {
+                const int r = tid & (PW - 1);
+                const int rs = min(r, jb - 1);
+                const int src = sm.src_top[rs];
+                const int dd = (r < jb) ? sm.down_dst[r] : -1;
 #pragma unroll 1
-            for (int t = tid; t < nright; t += kThreads) {
-                const int c = cb + t;
-                double *col = A + (size_t)c * lda + j0;
-                double x[PW], wv[PW];
+                for (int cl0 = tid / PW; cl0 < nright; cl0 += kCPI * kBatch) {
+                    double xv[kBatch], wv[kBatch];
 #pragma unroll
-                for (int r = 0; r < PW; ++r) x[r] = col[sm.src_top[min(r, jb - 1)]];
-#pragma unroll
-                for (int r = 0; r < PW; ++r) wv[r] = col[min(r, jb - 1)];
-#pragma unroll
-                for (int r = 0; r < PW; ++r) x[r] = (r < jb) ? x[r] : 0.0;
-                // the top rows that move below the panel
-#pragma unroll
-                for (int r = 0; r < PW; ++r) {
-                    const int d = (r < jb) ? sm.down_dst[r] : -1;
-                    if (d >= 0) col[d] = wv[r];
-                }
-                if (c >= cb) {
-                    // rows of L11 at and beyond jb are never read for k < jb
-                    // into a stored result: x[k] = 0 there
-                    if (!exact) {
-#pragma unroll
-                        for (int k = 0; k < PW; ++k) {
-                            const double u = -x[k];
-#pragma unroll
-                            for (int i = k + 1; i < PW; ++i) x[i] = fma(sm.L11[i][k], u, x[i]);
-                            __builtin_amdgcn_sched_barrier(0);
-                        }
-                    } else {
-#pragma unroll
-                        for (int k = 0; k < PW; ++k) {
-                            const double u = x[k];
-#pragma unroll
-                            for (int i = k + 1; i < PW; ++i)
-                                if (u != 0.0) x[i] = fma(sm.L11[i][k], -u, x[i]);
-                            __builtin_amdgcn_sched_barrier(0);
-                        }
+                    for (int u = 0; u < kBatch; ++u) {
+                        const double *col = A + (size_t)(cb + min(cl0 + kCPI * u, nright - 1)) * lda + j0;
+                        xv[u] = col[src];
+                        wv[u] = col[rs];
                     }
 #pragma unroll
-                    for (int k = 0; k < PW; ++k) sm.Us[k][c - cb] = (k < jb) ? -x[k] : -0.0;
+                    for (int u = 0; u < kBatch; ++u) {
+                        const int cl = cl0 + kCPI * u;
+                        if (cl < nright) {
+                            if (dd >= 0) A[(j0 + dd) + (size_t)(cb + cl) * lda] = wv[u];
+                            sm.Us[r][cl] = (r < jb) ? xv[u] : 0.0;
+                        }
+                    }
                 }
-#pragma unroll
-                for (int r = 0; r < PW; ++r)
-                    if (r < jb) col[r] = x[r];
             }
             __syncthreads();
+#pragma unroll 1
+            for (int cl = tid; cl < nright; cl += kThreads) {
+                double x[PW];
+#pragma unroll
+                for (int k = 0; k < PW; ++k) x[k] = sm.Us[k][cl];
+                // rows of L11 at and beyond jb are never read for k < jb
+                // into a stored result: x[k] = 0 there
+                if (!exact) {
+#pragma unroll
+                    for (int k = 0; k < PW; ++k) {
+                        const double u = -x[k];
+#pragma unroll
+                        for (int i = k + 1; i < PW; ++i) x[i] = fma(sm.L11[i][k], u, x[i]);
+                        __builtin_amdgcn_sched_barrier(0);
+                    }
+                } else {
+#pragma unroll
+                    for (int k = 0; k < PW; ++k) {
+                        const double u = x[k];
+#pragma unroll
+                        for (int i = k + 1; i < PW; ++i)
+                            if (u != 0.0) x[i] = fma(sm.L11[i][k], -u, x[i]);
+                        __builtin_amdgcn_sched_barrier(0);
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < PW; ++k) sm.Us[k][cl] = (k < jb) ? -x[k] : -0.0;
+            }
+            __syncthreads();
+            {
+                const int r = tid & (PW - 1);
+                if (r < jb) {
+#pragma unroll 1
+                    for (int cl0 = tid / PW; cl0 < nright; cl0 += kCPI * kBatch) {
+                        double v[kBatch];
+#pragma unroll
+                        for (int u = 0; u < kBatch; ++u) v[u] = sm.Us[r][min(cl0 + kCPI * u, nright - 1)];
+#pragma unroll
+                        for (int u = 0; u < kBatch; ++u)
+                            if (cl0 + kCPI * u < nright) A[(j0 + r) + (size_t)(cb + cl0 + kCPI * u) * lda] = -v[u];
+                    }
+                }
+            }
+            // no barrier: the update reads Us and rows >= cb only
 
             // ------------------------------------------------ update -------
             const int nrt = (m - cb + 31) / 32;

@@ -43,6 +43,7 @@
 
 #include "kernels.h"
 #include "tune.h"
+#include "wave_reduce.h"
 
 namespace magma_amd {
 namespace {
@@ -55,51 +56,8 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ int pick(const int *arr, int fixed, int id) { return arr ? arr[id] : fixed; }
 
-// (key, pos) order of the pivot search: larger key wins, ties to the smaller
-// logical position (first maximum).  Keys are never NaN.
-__device__ __forceinline__ void better(double &key, int &pos, double k2, int p2)
-{
-    const bool take = (k2 > key) | ((k2 == key) & (p2 < pos));
-    key = take ? k2 : key;
-    pos = take ? p2 : pos;
-}
-
-template <int CTRL>
-__device__ __forceinline__ void argmax_dpp(double &key, int &pos)
-{
-    const unsigned long long kb = (unsigned long long)__double_as_longlong(key);
-    const int lo = __builtin_amdgcn_update_dpp((int)kb, (int)kb, CTRL, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp((int)(kb >> 32), (int)(kb >> 32), CTRL, 0xF, 0xF, false);
-    const int p = __builtin_amdgcn_update_dpp(pos, pos, CTRL, 0xF, 0xF, false);
-    const double k2 = __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
-    better(key, pos, k2, p);
-}
-
-__device__ __forceinline__ double readlane_d(double v, int lane)
-{
-    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-    const unsigned lo = __builtin_amdgcn_readlane((int)b, lane);
-    const unsigned hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
-    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-
-// Wavefront arg-max of (key, pos); the result is wave-uniform.
-__device__ __forceinline__ void wave_argmax(double &key, int &pos)
-{
-    argmax_dpp<0xB1>(key, pos);    // quad_perm [1,0,3,2]
-    argmax_dpp<0x4E>(key, pos);    // quad_perm [2,3,0,1]
-    argmax_dpp<0x141>(key, pos);   // row_half_mirror
-    argmax_dpp<0x140>(key, pos);   // row_mirror: every lane holds its row's best
-    double k = readlane_d(key, 0);
-    int p = __builtin_amdgcn_readlane(pos, 0);
-#pragma unroll
-    for (int r = 1; r < 4; ++r) better(k, p, readlane_d(key, 16 * r), __builtin_amdgcn_readlane(pos, 16 * r));
-    key = k;
-    pos = p;
-}
-
 struct PivSlot {
-    double key;
+    unsigned long long key;   // pivot_key of the wavefront's best row
     int pos;
     int pad;
     double row[PW];
@@ -159,7 +117,10 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(M
 #pragma unroll
             for (int jj = 0; jj < PW; ++jj) {
                 if (jj < jb) {
-                    double key = -1.0;
+                    // ordered integer keys: |a| + 1 ulp of the bit pattern, 0 for
+                    // rows that are not candidates and for NaN past the first
+                    // position (idamax never takes those)
+                    unsigned long long key = 0;
                     int kp = INT_MAX;
                     int own = 0;
 #pragma unroll
@@ -167,16 +128,18 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(M
                         const int pr = tid + kThreads * i;
                         const bool cand = pr < mp && pos[i] >= jj;
                         const double v = a[i][jj];
-                        double k2 = -1.0;
-                        if (cand) k2 = (v != v) ? ((pos[i] == jj) ? __builtin_huge_val() : -1.0) : fabs(v);
-                        const double kold = key;
-                        const int pold = kp;
-                        better(key, kp, k2, cand ? pos[i] : INT_MAX);
-                        if (key != kold || kp != pold) own = i;
+                        unsigned long long k2 = 0;
+                        if (cand && (v == v || pos[i] == jj)) k2 = pivot_key(v == v ? fabs(v) : __builtin_huge_val());
+                        const int p2 = cand ? pos[i] : INT_MAX;
+                        if ((k2 > key) || (k2 == key && p2 < kp)) {
+                            key = k2;
+                            kp = p2;
+                            own = i;
+                        }
                     }
-                    double wk = key;
+                    unsigned long long wk = key;
                     int wp = kp;
-                    wave_argmax(wk, wp);
+                    wave_argmax_key(wk, wp);
                     PivSlot &sl = sm.slot[jj & 1][w];
                     if (wp != INT_MAX && kp == wp) {
 #pragma unroll
@@ -194,7 +157,7 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(M
                     __syncthreads();
                     // all four candidates, then the winner's row: two LDS
                     // round trips, each issued as one batch of reads
-                    double ck[kWaves];
+                    unsigned long long ck[kWaves];
                     int cp[kWaves];
 #pragma unroll
                     for (int q = 0; q < kWaves; ++q) {
@@ -202,7 +165,7 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(M
                         cp[q] = sm.slot[jj & 1][q].pos;
                     }
                     asm volatile("" ::: "memory");
-                    double bk = ck[0];
+                    unsigned long long bk = ck[0];
                     int bp = cp[0];
                     int bw = 0;
 #pragma unroll

@@ -40,11 +40,11 @@
 
 #include "kernels.h"
 #include "lu_reg.h"
+#include "wave_reduce.h"
 
 namespace magma_amd {
 namespace {
 
-using lu_reg_detail::group_argmax;
 using lu_reg_detail::readlane_f64;
 
 constexpr int kThreads = 256;
@@ -65,20 +65,23 @@ __device__ __forceinline__ void select_pivot(double &x0, double &x1, int pos0, i
                                              int lane, int *pivpos, double *pivval, double *col, int *ipiv)
 {
     const bool c0 = ok0 && pos0 >= jn, c1 = ok1 && pos1 >= jn;
-    double key = -1.0;
+    // ordered integer keys; NaN counts as the largest value at the first
+    // candidate position and is never taken elsewhere (idamax)
+    unsigned long long key = 0;
     int kp = INT_MAX;
     if (c0) {
-        key = (x0 != x0) ? ((pos0 == jn) ? __builtin_huge_val() : -1.0) : fabs(x0);
+        if (x0 == x0 || pos0 == jn) key = pivot_key(x0 == x0 ? fabs(x0) : __builtin_huge_val());
         kp = pos0;
     }
     if (c1) {
-        const double k1 = (x1 != x1) ? ((pos1 == jn) ? __builtin_huge_val() : -1.0) : fabs(x1);
+        const unsigned long long k1 =
+            (x1 == x1 || pos1 == jn) ? pivot_key(x1 == x1 ? fabs(x1) : __builtin_huge_val()) : 0ull;
         if ((k1 > key) | ((k1 == key) & (pos1 < kp))) {
             key = k1;
             kp = pos1;
         }
     }
-    group_argmax<64>(key, kp, lane);
+    wave_argmax_key(key, kp);
     const unsigned long long w0 = __builtin_amdgcn_ballot_w64(c0 && pos0 == kp);
     const unsigned long long w1 = __builtin_amdgcn_ballot_w64(c1 && pos1 == kp);
     const double src = w0 ? x0 : x1;

@@ -47,6 +47,7 @@
 #include "magma_amd.h"
 #include "mixed.h"
 #include "tune.h"
+#include "wave_reduce.h"
 
 namespace magma_amd {
 
@@ -71,7 +72,7 @@ constexpr int kUPad = CH + 4;   // staging row (halves): 72-byte stride, conflic
 __device__ __forceinline__ int pick(const int *arr, int fixed, int id) { return arr ? arr[id] : fixed; }
 
 struct FSlot {
-    float key;
+    unsigned key;     // pivot_key of the wavefront's best row
     int cur;
     int row;
     int pad;
@@ -142,38 +143,6 @@ __device__ __forceinline__ double wave_max(double v)
 #pragma unroll
     for (int r = 1; r < 4; ++r) m = dmax_nan(m, readlane_d(v, 16 * r));
     return m;
-}
-
-// arg-max key order: larger key, ties to the smaller current position
-__device__ __forceinline__ void better(float &key, int &cur, float k2, int c2)
-{
-    const bool take = (k2 > key) | ((k2 == key) & (c2 < cur));
-    key = take ? k2 : key;
-    cur = take ? c2 : cur;
-}
-
-template <int CTRL>
-__device__ __forceinline__ void argmax_step(float &key, int &cur)
-{
-    const float k2 = __int_as_float(dpp<CTRL>(__float_as_int(key)));
-    const int c2 = dpp<CTRL>(cur);
-    better(key, cur, k2, c2);
-}
-
-__device__ __forceinline__ void wave_argmax(float &key, int &cur)
-{
-    argmax_step<0xB1>(key, cur);
-    argmax_step<0x4E>(key, cur);
-    argmax_step<0x141>(key, cur);
-    argmax_step<0x140>(key, cur);
-    float k = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(key), 0));
-    int c = __builtin_amdgcn_readlane(cur, 0);
-#pragma unroll
-    for (int r = 1; r < 4; ++r)
-        better(k, c, __int_as_float(__builtin_amdgcn_readlane(__float_as_int(key), 16 * r)),
-               __builtin_amdgcn_readlane(cur, 16 * r));
-    key = k;
-    cur = c;
 }
 
 template <int I, int N, class Fn>
@@ -492,22 +461,22 @@ struct Mixed {
         for (int jj = 0; jj < CH; ++jj) {
             if (jj < kb) {
                 const int gpos = c0 + s0 + jj;
-                float key = -1.0f;
+                unsigned key = 0;                    // ordered integer key, 0: no candidate
                 int kc = INT_MAX, own = -1;
 #pragma unroll
                 for (int r = 0; r < RPT; ++r) {
                     if (act[r]) {
                         const float v = a[r][jj];
-                        const float k2 = (v != v) ? __builtin_huge_valf() : fabsf(v);
+                        const unsigned k2 = pivot_key((v != v) ? __builtin_huge_valf() : fabsf(v));
                         const bool take = (k2 > key) | ((k2 == key) & (cur[r] < kc));
                         key = take ? k2 : key;
                         kc = take ? cur[r] : kc;
                         own = take ? r : own;
                     }
                 }
-                float wk = key;
+                unsigned wk = key;
                 int wc = kc;
-                wave_argmax(wk, wc);
+                wave_argmax_key(wk, wc);
                 FSlot &sl = sm.slot[jj & 1][w];
                 if (own >= 0 && kc == wc) {
 #pragma unroll
@@ -524,7 +493,7 @@ struct Mixed {
                     sl.cur = wc;
                 }
                 __syncthreads();
-                float ck[kWaves];
+                unsigned ck[kWaves];
                 int cc4[kWaves];
 #pragma unroll
                 for (int q = 0; q < kWaves; ++q) {
@@ -532,7 +501,7 @@ struct Mixed {
                     cc4[q] = sm.slot[jj & 1][q].cur;
                 }
                 asm volatile("" ::: "memory");
-                float bk = ck[0];
+                unsigned bk = ck[0];
                 int bc = cc4[0], bw = 0;
 #pragma unroll
                 for (int q = 1; q < kWaves; ++q) {

@@ -1,0 +1,64 @@
+// wave_reduce.h -- wavefront-wide integer max / min and the (key, position)
+// arg-max of the pivot searches.
+//
+// A reduction is a prefix scan within each row of 16 lanes by DPP (row_shr 1,
+// 2, 4, 8: one integer max or min per step, the shifted-in lanes reading the
+// identity) followed by the four row ends (lanes 15, 31, 47, 63) read with
+// readlane and combined on the scalar unit.  Pivot keys are the bit patterns
+// of non-negative floating values plus one (0 marks a row that is not a
+// candidate): unsigned comparison of those orders them as the values, so the
+// arg-max needs integer steps only, where a (float or double key, position)
+// pair needed a compare, a tie test and two selects per step.  The arg-max is
+// the largest key, ties to the smallest position: idamax's first maximum when
+// positions are the rows' current logical positions.
+#pragma once
+
+#include <climits>
+
+namespace magma_amd {
+
+__device__ __forceinline__ unsigned wave_max_u32(unsigned v)
+{
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false));   // row_shr:1
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false));   // row_shr:2
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false));   // row_shr:4
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false));   // row_shr:8
+    const unsigned a = __builtin_amdgcn_readlane(v, 15), b = __builtin_amdgcn_readlane(v, 31);
+    const unsigned c = __builtin_amdgcn_readlane(v, 47), d = __builtin_amdgcn_readlane(v, 63);
+    return max(max(a, b), max(c, d));
+}
+
+__device__ __forceinline__ int wave_min_i32(int v)
+{
+    v = min(v, __builtin_amdgcn_update_dpp(INT_MAX, v, 0x111, 0xF, 0xF, false));
+    v = min(v, __builtin_amdgcn_update_dpp(INT_MAX, v, 0x112, 0xF, 0xF, false));
+    v = min(v, __builtin_amdgcn_update_dpp(INT_MAX, v, 0x114, 0xF, 0xF, false));
+    v = min(v, __builtin_amdgcn_update_dpp(INT_MAX, v, 0x118, 0xF, 0xF, false));
+    const int a = __builtin_amdgcn_readlane(v, 15), b = __builtin_amdgcn_readlane(v, 31);
+    const int c = __builtin_amdgcn_readlane(v, 47), d = __builtin_amdgcn_readlane(v, 63);
+    return min(min(a, b), min(c, d));
+}
+
+// ordered pivot key of |a| (a non-negative, +inf allowed)
+__device__ __forceinline__ unsigned long long pivot_key(double a) { return (unsigned long long)__double_as_longlong(a) + 1ull; }
+__device__ __forceinline__ unsigned pivot_key(float a) { return (unsigned)__float_as_int(a) + 1u; }
+
+// (key, pos) over the wavefront: largest key, ties to the smallest pos.  The
+// result is wave-uniform.
+__device__ __forceinline__ void wave_argmax_key(unsigned long long &key, int &pos)
+{
+    const unsigned hi = (unsigned)(key >> 32), lo = (unsigned)key;
+    const unsigned mh = wave_max_u32(hi);
+    const unsigned ml = wave_max_u32(hi == mh ? lo : 0u);
+    pos = wave_min_i32((hi == mh && lo == ml) ? pos : INT_MAX);
+    key = ((unsigned long long)mh << 32) | ml;
+}
+
+__device__ __forceinline__ void wave_argmax_key(unsigned &key, int &pos)
+{
+    const unsigned m = wave_max_u32(key);
+    pos = wave_min_i32(key == m ? pos : INT_MAX);
+    key = m;
+}
+
+}  // namespace magma_amd

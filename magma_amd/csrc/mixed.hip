@@ -493,36 +493,41 @@ struct Mixed {
                     sl.cur = wc;
                 }
                 __syncthreads();
+                // each slot's pivot value is read with the keys, so the
+                // reciprocal starts while the winner's row is in flight
                 unsigned ck[kWaves];
                 int cc4[kWaves];
+                float cv[kWaves];
 #pragma unroll
                 for (int q = 0; q < kWaves; ++q) {
                     ck[q] = sm.slot[jj & 1][q].key;
                     cc4[q] = sm.slot[jj & 1][q].cur;
+                    cv[q] = sm.slot[jj & 1][q].val[jj];
                 }
                 asm volatile("" ::: "memory");
                 unsigned bk = ck[0];
                 int bc = cc4[0], bw = 0;
+                float piv = cv[0];
 #pragma unroll
                 for (int q = 1; q < kWaves; ++q) {
                     const bool take = (ck[q] > bk) | ((ck[q] == bk) & (cc4[q] < bc));
                     bk = take ? ck[q] : bk;
                     bc = take ? cc4[q] : bc;
                     bw = take ? q : bw;
+                    piv = take ? cv[q] : piv;
                 }
+                const float rc = 1.0f / piv;
                 const FSlot &ps = sm.slot[jj & 1][bw];
                 const int prow = ps.row;
                 float u[CH];
 #pragma unroll
                 for (int cc = jj; cc < CH; ++cc) u[cc] = ps.val[cc];
                 asm volatile("" ::: "memory");
-                const float piv = u[jj];
                 if (!(piv != 0.0f) || piv != piv) bad = 1;
                 if (tid == 0) {
                     sm.p[gpos] = prow;
                     if (ipiv) ipiv[gpos] = bc + 1;
                 }
-                const float rc = 1.0f / piv;
 #pragma unroll
                 for (int r = 0; r < RPT; ++r) {
                     const int o = tid + kT * r;

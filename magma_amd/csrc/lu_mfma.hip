@@ -56,11 +56,11 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ int pick(const int *arr, int fixed, int id) { return arr ? arr[id] : fixed; }
 
-struct PivSlot {
+struct alignas(16) PivSlot {
     unsigned long long key;   // pivot_key of the wavefront's best row
     int pos;
     int pad;
-    double row[PW];
+    double row[PW];           // 16-byte aligned: written in pairs
 };
 
 template <int MT>
@@ -142,12 +142,18 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(M
                     wave_argmax_key(wk, wp);
                     PivSlot &sl = sm.slot[jj & 1][w];
                     if (wp != INT_MAX && kp == wp) {
+                        // 16-byte stores from the pair holding column jj on
+                        // (the entries left of jj are written too, and unread)
+                        double2 *dst = reinterpret_cast<double2 *>(sl.row);
 #pragma unroll
-                        for (int c = jj; c < PW; ++c) {
-                            double v = a[0][c];
+                        for (int c = jj & ~1; c < PW; c += 2) {
+                            double v0 = a[0][c], v1 = a[0][c + 1];
 #pragma unroll
-                            for (int i = 1; i < RPT; ++i) v = (own == i) ? a[i][c] : v;
-                            sl.row[c] = v;
+                            for (int i = 1; i < RPT; ++i) {
+                                v0 = (own == i) ? a[i][c] : v0;
+                                v1 = (own == i) ? a[i][c + 1] : v1;
+                            }
+                            dst[c / 2] = make_double2(v0, v1);
                         }
                     }
                     if (lane == 0) {
@@ -180,7 +186,11 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(M
                     // the panel (one panel row per thread); else read per use
                     double u[PW];
 #pragma unroll
-                    for (int c = jj; c < PW; ++c) u[c] = prow[c];
+                    for (int c = jj & ~1; c < PW; c += 2) {
+                        const double2 t = reinterpret_cast<const double2 *>(prow)[c / 2];
+                        u[c] = t.x;
+                        u[c + 1] = t.y;
+                    }
                     if (RPT == 1) asm volatile("" ::: "memory");
                     const int p = bp;
                     if (tid == 0) sm.swaps[jj] = p;

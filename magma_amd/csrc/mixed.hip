@@ -71,12 +71,12 @@ constexpr int kUPad = CH + 4;   // staging row (halves): 72-byte stride, conflic
 
 __device__ __forceinline__ int pick(const int *arr, int fixed, int id) { return arr ? arr[id] : fixed; }
 
-struct FSlot {
+struct alignas(16) FSlot {
     unsigned key;     // pivot_key of the wavefront's best row
     int cur;
     int row;
     int pad;
-    float val[CH];
+    float val[CH];    // 16-byte aligned: written and read four at a time
 };
 
 template <int NMAX, int NB, int RG>
@@ -479,12 +479,19 @@ struct Mixed {
                 wave_argmax_key(wk, wc);
                 FSlot &sl = sm.slot[jj & 1][w];
                 if (own >= 0 && kc == wc) {
+                    // 16-byte stores from the quad holding column jj on (the
+                    // entries left of jj are written too, and unread)
+                    float4 *dst = reinterpret_cast<float4 *>(sl.val);
 #pragma unroll
-                    for (int cc = jj; cc < CH; ++cc) {
-                        float v = a[0][cc];
+                    for (int cc = jj & ~3; cc < CH; cc += 4) {
+                        float v[4];
 #pragma unroll
-                        for (int r = 1; r < RPT; ++r) v = (own == r) ? a[r][cc] : v;
-                        sl.val[cc] = v;
+                        for (int e = 0; e < 4; ++e) {
+                            v[e] = a[0][cc + e];
+#pragma unroll
+                            for (int r = 1; r < RPT; ++r) v[e] = (own == r) ? a[r][cc + e] : v[e];
+                        }
+                        dst[cc / 4] = make_float4(v[0], v[1], v[2], v[3]);
                     }
                     sl.row = tid + kT * own;
                 }
@@ -521,7 +528,13 @@ struct Mixed {
                 const int prow = ps.row;
                 float u[CH];
 #pragma unroll
-                for (int cc = jj; cc < CH; ++cc) u[cc] = ps.val[cc];
+                for (int cc = jj & ~3; cc < CH; cc += 4) {
+                    const float4 t = reinterpret_cast<const float4 *>(ps.val)[cc / 4];
+                    u[cc] = t.x;
+                    u[cc + 1] = t.y;
+                    u[cc + 2] = t.z;
+                    u[cc + 3] = t.w;
+                }
                 asm volatile("" ::: "memory");
                 if (!(piv != 0.0f) || piv != piv) bad = 1;
                 if (tid == 0) {

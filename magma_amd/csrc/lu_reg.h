@@ -30,6 +30,7 @@
 #include <climits>
 
 #include "kernels.h"
+#include "wave_reduce.h"
 
 namespace magma_amd {
 namespace lu_reg_detail {
@@ -47,62 +48,11 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// (key, pos) step of the pivot search across lanes by DPP: larger key wins,
-// ties to the smaller logical position (idamax's first maximum).
-template <int CTRL>
-__device__ __forceinline__ void amax_dpp(double &key, int &kp)
-{
-    const unsigned long long b = (unsigned long long)__double_as_longlong(key);
-    const int lo = __builtin_amdgcn_update_dpp((int)b, (int)b, CTRL, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp((int)(b >> 32), (int)(b >> 32), CTRL, 0xF, 0xF, false);
-    const int p2 = __builtin_amdgcn_update_dpp(kp, kp, CTRL, 0xF, 0xF, false);
-    const double k2 = __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
-    const bool take = (k2 > key) | ((k2 == key) & (p2 < kp));
-    key = take ? k2 : key;
-    kp = take ? p2 : kp;
-}
-
 __device__ __forceinline__ double readlane_f64(double v, int l)
 {
     const unsigned long long b = (unsigned long long)__double_as_longlong(v);
     const unsigned lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-
-// Arg-max of (key, pos) over each group of NMAX lanes (8, 16, 32 or 64); every
-// lane of a group ends with its group's result.  Quad swaps, the half-row and
-// row mirrors reduce within a row of 16 lanes; the row leaders are combined
-// with readlane.
-template <int NMAX>
-__device__ __forceinline__ void group_argmax(double &key, int &kp, int lane)
-{
-    amax_dpp<0xB1>(key, kp);
-    amax_dpp<0x4E>(key, kp);
-    amax_dpp<0x141>(key, kp);
-    if constexpr (NMAX >= 16) amax_dpp<0x140>(key, kp);
-    if constexpr (NMAX >= 32) {
-        double k[4];
-        int p[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            k[q] = readlane_f64(key, 16 * q);
-            p[q] = __builtin_amdgcn_readlane(kp, 16 * q);
-        }
-#pragma unroll
-        for (int q = 0; q < 4; q += 2) {
-            const bool take = (k[q + 1] > k[q]) | ((k[q + 1] == k[q]) & (p[q + 1] < p[q]));
-            k[q] = take ? k[q + 1] : k[q];
-            p[q] = take ? p[q + 1] : p[q];
-        }
-        if constexpr (NMAX == 64) {
-            const bool take = (k[2] > k[0]) | ((k[2] == k[0]) & (p[2] < p[0]));
-            key = take ? k[2] : k[0];
-            kp = take ? p[2] : p[0];
-        } else {
-            key = lane < 32 ? k[0] : k[2];
-            kp = lane < 32 ? p[0] : p[2];
-        }
-    }
 }
 
 // Minimum resident waves per SIMD asked of the register allocator.
@@ -178,21 +128,19 @@ __device__ __forceinline__ void row_update(double (&a)[NMAX], int jj, double l, 
     }
 }
 
-// The same update for NMAX = 64 (one matrix per wavefront) with the operand of
-// column k taken from another lane by readlane instead of LDS: from register
-// k of lane src (FROM_ROW: the LU pivot row, not updated in this column), or
-// from value v of lane k (the Cholesky multiplier of row k).  jj is a
-// constant after the caller's column loop is unrolled.
-template <int NMAX, bool EXACT, bool NEGU_FIRST, bool FROM_ROW>
-__device__ __forceinline__ void row_update_lanes(double (&a)[NMAX], int jj, double l, int src, double v)
+// The same update for NMAX = 64 (one matrix per wavefront) with u_k taken from
+// register k of the pivot row's lane src by readlane instead of LDS (at n = 64
+// the LDS hand-off of the pivot row measured slower: 0.78 against 0.99 ms for
+// 20,000 factorizations).  jj is a constant after the caller's column loop is
+// unrolled.
+template <int NMAX, bool EXACT>
+__device__ __forceinline__ void row_update_from_lane(double (&a)[NMAX], int jj, double l, int src)
 {
 #pragma unroll
     for (int k = 0; k < NMAX; ++k) {
         if (k <= jj) continue;
-        const double uk = FROM_ROW ? readlane_f64(a[k], src) : readlane_f64(v, k);
-        double r;
-        if constexpr (NEGU_FIRST) r = fma(-uk, l, a[k]);
-        else r = fma(l, -uk, a[k]);
+        const double uk = readlane_f64(a[k], src);
+        const double r = fma(l, -uk, a[k]);
         if constexpr (EXACT) a[k] = (uk != 0.0) ? r : a[k];
         else a[k] = r;
     }
@@ -325,13 +273,17 @@ __attribute__((amdgpu_waves_per_eu(reg_waves_per_eu<NMAX>()))) getrf_reg_kernel(
                     // pivot search: first maximum of |a(pos>=j, j)| (netlib idamax)
                     const double v = a[jj];
                     const bool cand = rowok && pos >= j;
-                    double key = -1.0;
-                    if (cand) key = (v != v) ? ((pos == j) ? __builtin_huge_val() : -1.0) : fabs(v);
+                    // ordered integer key (0: not a candidate, or NaN past the
+                    // first position, which idamax never takes)
+                    unsigned long long key = 0;
+                    if (cand && (v == v || pos == j)) key = pivot_key(v == v ? fabs(v) : __builtin_huge_val());
                     int kp = cand ? pos : INT_MAX;
-                    group_argmax<NMAX>(key, kp, lane);
+                    group_argmax_key<NMAX>(key, kp, lane);
                     const int p = kp;
+                    // the pivot row: by readlane from its lane (n = 64), else
+                    // through LDS (its lane writes, all read)
                     double piv;
-                    int ol = 0;                              // lane of the pivot row (NMAX = 64)
+                    int ol = 0;
                     if constexpr (NMAX == 64) {
                         const unsigned long long ow = __builtin_amdgcn_ballot_w64(cand && pos == p);
                         ol = ow ? (int)__builtin_ctzll(ow) : 0;
@@ -363,8 +315,8 @@ __attribute__((amdgpu_waves_per_eu(reg_waves_per_eu<NMAX>()))) getrf_reg_kernel(
                         // entry is exactly zero; that changes the result only when a
                         // multiplier is Inf/NaN, so the exact form runs only then.
                         if constexpr (NMAX == 64) {
-                            if (!__builtin_isfinite(l)) row_update_lanes<NMAX, true, false, true>(a, jj, l, ol, 0.0);
-                            else row_update_lanes<NMAX, false, false, true>(a, jj, l, ol, 0.0);
+                            if (!__builtin_isfinite(l)) row_update_from_lane<NMAX, true>(a, jj, l, ol);
+                            else row_update_from_lane<NMAX, false>(a, jj, l, ol);
                         } else {
                             if (!__builtin_isfinite(l)) row_update<NMAX, true, false>(a, jj, l, buf, kmax);
                             else row_update<NMAX, false, false>(a, jj, l, buf, kmax);

@@ -61,4 +61,49 @@ __device__ __forceinline__ void wave_argmax_key(unsigned &key, int &pos)
     key = m;
 }
 
+// The same within groups of G lanes (8, 16, 32 or 64; several matrices per
+// wavefront): quad swaps, the half-row and row mirrors, then the row values
+// by readlane.  Every lane gets its group's result.
+template <int G>
+__device__ __forceinline__ unsigned group_max_u32(unsigned v, int lane)
+{
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false));    // quad_perm [1,0,3,2]
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false));    // quad_perm [2,3,0,1]
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, false));   // row_half_mirror
+    if constexpr (G >= 16) v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xF, 0xF, false));
+    if constexpr (G >= 32) {
+        const unsigned r0 = __builtin_amdgcn_readlane(v, 0), r1 = __builtin_amdgcn_readlane(v, 16);
+        const unsigned r2 = __builtin_amdgcn_readlane(v, 32), r3 = __builtin_amdgcn_readlane(v, 48);
+        if constexpr (G == 64) v = max(max(r0, r1), max(r2, r3));
+        else v = lane < 32 ? max(r0, r1) : max(r2, r3);
+    }
+    return v;
+}
+
+template <int G>
+__device__ __forceinline__ int group_min_i32(int v, int lane)
+{
+    v = min(v, __builtin_amdgcn_update_dpp(INT_MAX, v, 0xB1, 0xF, 0xF, false));
+    v = min(v, __builtin_amdgcn_update_dpp(INT_MAX, v, 0x4E, 0xF, 0xF, false));
+    v = min(v, __builtin_amdgcn_update_dpp(INT_MAX, v, 0x141, 0xF, 0xF, false));
+    if constexpr (G >= 16) v = min(v, __builtin_amdgcn_update_dpp(INT_MAX, v, 0x140, 0xF, 0xF, false));
+    if constexpr (G >= 32) {
+        const int r0 = __builtin_amdgcn_readlane(v, 0), r1 = __builtin_amdgcn_readlane(v, 16);
+        const int r2 = __builtin_amdgcn_readlane(v, 32), r3 = __builtin_amdgcn_readlane(v, 48);
+        if constexpr (G == 64) v = min(min(r0, r1), min(r2, r3));
+        else v = lane < 32 ? min(r0, r1) : min(r2, r3);
+    }
+    return v;
+}
+
+template <int G>
+__device__ __forceinline__ void group_argmax_key(unsigned long long &key, int &pos, int lane)
+{
+    const unsigned hi = (unsigned)(key >> 32), lo = (unsigned)key;
+    const unsigned mh = group_max_u32<G>(hi, lane);
+    const unsigned ml = group_max_u32<G>(hi == mh ? lo : 0u, lane);
+    pos = group_min_i32<G>((hi == mh && lo == ml) ? pos : INT_MAX, lane);
+    key = ((unsigned long long)mh << 32) | ml;
+}
+
 }  // namespace magma_amd

@@ -44,11 +44,21 @@ __device__ __forceinline__ unsigned long long pivot_key(double a) { return (unsi
 __device__ __forceinline__ unsigned pivot_key(float a) { return (unsigned)__float_as_int(a) + 1u; }
 
 // (key, pos) over the wavefront: largest key, ties to the smallest pos.  The
-// result is wave-uniform.
+// result is wave-uniform.  When one lane alone holds the largest high word
+// (the usual case: equal high words need equal exponents and 20 leading
+// mantissa bits) that lane is the arg-max and the low-word and position
+// passes are skipped.
 __device__ __forceinline__ void wave_argmax_key(unsigned long long &key, int &pos)
 {
     const unsigned hi = (unsigned)(key >> 32), lo = (unsigned)key;
     const unsigned mh = wave_max_u32(hi);
+    const unsigned long long tie = __builtin_amdgcn_ballot_w64(hi == mh);
+    if (__builtin_popcountll(tie) == 1) {
+        const int l = (int)__builtin_ctzll(tie);
+        pos = __builtin_amdgcn_readlane(pos, l);
+        key = ((unsigned long long)mh << 32) | (unsigned)__builtin_amdgcn_readlane((int)lo, l);
+        return;
+    }
     const unsigned ml = wave_max_u32(hi == mh ? lo : 0u);
     pos = wave_min_i32((hi == mh && lo == ml) ? pos : INT_MAX);
     key = ((unsigned long long)mh << 32) | ml;
@@ -57,7 +67,12 @@ __device__ __forceinline__ void wave_argmax_key(unsigned long long &key, int &po
 __device__ __forceinline__ void wave_argmax_key(unsigned &key, int &pos)
 {
     const unsigned m = wave_max_u32(key);
-    pos = wave_min_i32(key == m ? pos : INT_MAX);
+    const unsigned long long tie = __builtin_amdgcn_ballot_w64(key == m);
+    if (__builtin_popcountll(tie) == 1) {
+        pos = __builtin_amdgcn_readlane(pos, (int)__builtin_ctzll(tie));
+    } else {
+        pos = wave_min_i32(key == m ? pos : INT_MAX);
+    }
     key = m;
 }
 
@@ -101,6 +116,21 @@ __device__ __forceinline__ void group_argmax_key(unsigned long long &key, int &p
 {
     const unsigned hi = (unsigned)(key >> 32), lo = (unsigned)key;
     const unsigned mh = group_max_u32<G>(hi, lane);
+    if constexpr (G >= 32) {
+        // one lane per group with the largest high word: that lane is the
+        // arg-max (as in wave_argmax_key)
+        const unsigned long long tie = __builtin_amdgcn_ballot_w64(hi == mh);
+        const unsigned t0 = (unsigned)tie, t1 = (unsigned)(tie >> 32);
+        if (G == 64 ? __builtin_popcountll(tie) == 1 : (__builtin_popcount(t0) == 1 && __builtin_popcount(t1) == 1)) {
+            const int l0 = (G == 64) ? (int)__builtin_ctzll(tie) : (int)__builtin_ctz(t0);
+            const int l1 = (G == 64) ? l0 : 32 + (int)__builtin_ctz(t1);
+            const int p0 = __builtin_amdgcn_readlane(pos, l0), p1 = __builtin_amdgcn_readlane(pos, l1);
+            const unsigned q0 = __builtin_amdgcn_readlane((int)lo, l0), q1 = __builtin_amdgcn_readlane((int)lo, l1);
+            pos = lane < 32 ? p0 : p1;
+            key = ((unsigned long long)mh << 32) | (lane < 32 ? q0 : q1);
+            return;
+        }
+    }
     const unsigned ml = group_max_u32<G>(hi == mh ? lo : 0u, lane);
     pos = group_min_i32<G>((hi == mh && lo == ml) ? pos : INT_MAX, lane);
     key = ((unsigned long long)mh << 32) | ml;

@@ -79,7 +79,14 @@ magma_int_t magma_dsposv_vbatched(magma_uplo_t uplo, magma_int_t *n, magma_int_t
  * control/get_batched_gemm_decision.cpp).  magma_amd_autotune times every
  * variant of every family on the queue's device with synthetic matrices of
  * each size class and keeps the fastest.  Untuned entries use variant 0.
- * Families: 0 LU (64 < n <= 512), 1 mixed-precision solver, 2 DGEMM tiles.
+ * Families and their variants:
+ *   0 LU, 64 < n <= 512: 0 MFMA kernel, 1 the same at one workgroup per CU,
+ *     2 VALU blocked kernel, 3 / 4 register-resident kernel at three / two
+ *     workgroups per CU (size class 4, n <= 128, only);
+ *   1 mixed-precision solver: five variants for n <= 256, two above;
+ *   2 DGEMM: 0 32x32, 1 16x16, 2 48x48 tiles per wavefront, 3 64x64 tiles
+ *     per workgroup staged through LDS.
+ * magma_amd_tune_set returns -3 for a variant the size class does not have.
  * Size classes: 0..8 for max dimension <= 8, 16, 32, ..., 2048; 9 = larger
  * or unknown (vbatched calls without tight bounds). */
 #define MAGMA_AMD_TUNE_GETRF   1

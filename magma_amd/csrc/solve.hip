@@ -121,6 +121,12 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
         // after a fused factorization: no solve with a singular factor (dgesv/dposv)
         if (s.info && s.info[id] != 0) continue;
 
+        // the interchanges staged in LDS (after the right-hand-side chunk), so
+        // the serial dlaswp chain below waits on LDS, not on a global load per step
+        int *piv = reinterpret_cast<int *>(X + (size_t)n * chunk);
+        if (ipiv) {
+            for (int k = tid; k < n; k += kThreads) piv[k] = ipiv[k];
+        }
         for (int c0 = 0; c0 < nrhs; c0 += chunk) {
             const int nc = min(chunk, nrhs - c0);
             for (int e = tid; e < n * nc; e += kThreads) {
@@ -132,7 +138,7 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
                 // dlaswp forward: interchanges in order, one thread per column
                 if (tid < nc)
                     for (int k = 0; k < n; ++k) {
-                        const int p = ipiv[k] - 1;
+                        const int p = piv[k] - 1;
                         if (p != k && p >= 0 && p < n) {
                             double *x = X + (size_t)tid * ldx;
                             const double t = x[k];
@@ -148,7 +154,7 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
                 sweep(A, lda, n, X, ldx, nc, true, true, true);
                 if (tid < nc)
                     for (int k = n - 1; k >= 0; --k) {
-                        const int p = ipiv[k] - 1;
+                        const int p = piv[k] - 1;
                         if (p != k && p >= 0 && p < n) {
                             double *x = X + (size_t)tid * ldx;
                             const double t = x[k];
@@ -182,7 +188,7 @@ static magma_int_t launch_solve(const MatSet &s, char mode, int max_n, hipStream
     // right-hand-side columns per pass: as many as fit 64 KB of LDS, at most kChunk
     const int chunk = std::min<long>(kChunk, (64 * 1024) / (sizeof(double) * (size_t)max_n));
     if (chunk < 1) return MAGMA_ERR_NOT_SUPPORTED;
-    const size_t lds = sizeof(double) * (size_t)max_n * chunk;
+    const size_t lds = sizeof(double) * (size_t)max_n * chunk + sizeof(int) * (size_t)max_n;
     const int grid = s.dev_count ? persistent_grid(s.count, 4) : s.count;
     solve_kernel<<<grid, kThreads, lds, st>>>(s, mode, chunk);
     MAGMA_LAUNCH_CHECK();

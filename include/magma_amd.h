@@ -56,7 +56,7 @@ magma_int_t magma_dsposv_batched(magma_uplo_t uplo, magma_int_t n, magma_int_t n
  * codes as the two above, with n, nrhs and the leading dimensions in device
  * arrays of batchCount+1 entries (the vbatched convention of
  * include/magma_zvbatched.h).  Orders above 512 take the FP64 path with
- * iter = -1; orders above 2048 return MAGMA_ERR_NOT_SUPPORTED.  The checked
+ * iter = -1; orders above 8192 return MAGMA_ERR_NOT_SUPPORTED.  The checked
  * vbatched calls read the checker's flags back, so they synchronize. */
 magma_int_t magma_dsgesv_vbatched(magma_int_t *n, magma_int_t *nrhs,
                                   double **dA_array, magma_int_t *ldda, magma_int_t **dipiv_array,
@@ -88,7 +88,8 @@ magma_int_t magma_dsposv_vbatched(magma_uplo_t uplo, magma_int_t *n, magma_int_t
  *     per workgroup staged through LDS.
  * magma_amd_tune_set returns -3 for a variant the size class does not have.
  * Size classes: 0..8 for max dimension <= 8, 16, 32, ..., 2048; 9 = larger
- * or unknown (vbatched calls without tight bounds). */
+ * (up to 8192 for the factorizations) or unknown (vbatched calls without
+ * tight bounds). */
 #define MAGMA_AMD_TUNE_GETRF   1
 #define MAGMA_AMD_TUNE_MIXED   2
 #define MAGMA_AMD_TUNE_GEMM    4

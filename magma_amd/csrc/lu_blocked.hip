@@ -468,6 +468,8 @@ magma_int_t launch_getrf_blocked(const MatSet &s, int max_m, hipStream_t st)
     else if (max_m <= 512)  getrf_blocked_kernel<2, 16><<<grid, kThreads, 0, st>>>(s);
     else if (max_m <= 1024) getrf_blocked_kernel<4, 8><<<grid, kThreads, 0, st>>>(s);
     else if (max_m <= 2048) getrf_blocked_kernel<8, 4><<<grid, kThreads, 0, st>>>(s);
+    else if (max_m <= 4096) getrf_blocked_kernel<16, 2><<<grid, kThreads, 0, st>>>(s);
+    else if (max_m <= 8192) getrf_blocked_kernel<32, 1><<<grid, kThreads, 0, st>>>(s);
     else return MAGMA_ERR_NOT_SUPPORTED;
     MAGMA_LAUNCH_CHECK();
     return MAGMA_SUCCESS;
@@ -481,6 +483,8 @@ magma_int_t launch_potrf_blocked(const MatSet &s, int max_n, hipStream_t st)
     else if (max_n <= 512)  potrf_blocked_kernel<2, 16><<<grid, kThreads, 0, st>>>(s);
     else if (max_n <= 1024) potrf_blocked_kernel<4, 8><<<grid, kThreads, 0, st>>>(s);
     else if (max_n <= 2048) potrf_blocked_kernel<8, 4><<<grid, kThreads, 0, st>>>(s);
+    else if (max_n <= 4096) potrf_blocked_kernel<16, 2><<<grid, kThreads, 0, st>>>(s);
+    else if (max_n <= 8192) potrf_blocked_kernel<32, 1><<<grid, kThreads, 0, st>>>(s);
     else return MAGMA_ERR_NOT_SUPPORTED;
     MAGMA_LAUNCH_CHECK();
     return MAGMA_SUCCESS;

@@ -197,7 +197,9 @@ magma_int_t run_classes_device(char kind, MatSet s, const VScratch &v, int max_d
     return MAGMA_SUCCESS;
 }
 
-constexpr int kMaxFactorDim = 2048;
+// Largest order of the factor kernels (above 2048 the blocked kernels narrow
+// their panels to keep the panel column in registers).
+constexpr int kMaxFactorDim = 8192;
 
 // Appends ids[0 .. cnt) to a fallback list with dsgesv's iter code -1 (the
 // order is beyond the mixed kernels).
@@ -273,7 +275,7 @@ magma_int_t mixed_vbatched(char kind, bool upper, int *n, int *nrhs, double **dA
         }
     }
     const int n_mid = counts[6];
-    const int n_large = counts[7] + counts[8];
+    const int n_large = counts[7] + counts[8] + counts[9];
     // factor slots and status entries: each launch has its own plan; the
     // launches are ordered on the stream, so they share one region
     size_t slot_bytes = 0;

@@ -6,9 +6,10 @@
 namespace magma_amd {
 
 // Size classes by max(m, n): <=8, <=16, <=32, <=64, <=128, <=256, <=512,
-// <=1024, <=2048, larger.
+// <=1024, <=2048, <=8192 (the largest order the factor kernels take; callers
+// reject larger ones before binning).
 constexpr int kNumBuckets = 10;
-constexpr int kBucketLimit[kNumBuckets] = {8, 16, 32, 64, 128, 256, 512, 1024, 2048, 1 << 30};
+constexpr int kBucketLimit[kNumBuckets] = {8, 16, 32, 64, 128, 256, 512, 1024, 2048, 8192};
 
 // Views into one device allocation of workqueue_bytes(count):
 //   ids[count]            matrix ids sorted by size class

@@ -337,5 +337,5 @@ def test_mixed_vbatched_argument_errors(queue):
     assert L.magma_dsposv_vbatched(0, p(ok), p(ok), 0, p(ok), 0, p(ok), 0, p(ok), 0, 0, 3, q) == -1      # no such uplo
     assert L.magma_dsposv_vbatched(ma.MagmaLower, p(neg), p(ok), 0, p(ok), 0, p(ok), 0, p(ok), 0, 0, 3, q) == -1
     assert L.magma_dsposv_vbatched(ma.MagmaLower, p(ok), p(ok), 0, p(small), 0, p(ok), 0, p(ok), 0, 0, 3, q) == -4
-    big = ma.dev_ints([4, 3000, 4])
+    big = ma.dev_ints([4, 9000, 4])
     assert sg(n=big, la=big, lb=big, lx=big) == ma_lib.MAGMA_ERR_NOT_SUPPORTED

@@ -57,6 +57,14 @@ __device__ __forceinline__ double readlane_f64(double v, int l)
 
 // Minimum resident waves per SIMD asked of the register allocator.
 template <int NMAX> constexpr int reg_waves_per_eu() { return NMAX >= 64 ? 2 : 4; }
+// The n = 64 Cholesky of fixed-size batches fits three (168 registers, a few
+// spilled outside the column loops): 20,000 posv with 4 right-hand sides
+// measured 0.87 -> 0.75 ms against two.  The variable-size form spills
+// heavily at three and keeps two.
+template <int NMAX, bool VAR> constexpr int potrf_waves_per_eu()
+{
+    return (NMAX >= 64 && !VAR) ? 3 : reg_waves_per_eu<NMAX>();
+}
 
 // Columns per unrolled chunk.  All columns are unrolled: a rotated frame
 // (chunks of 8 with the row rotated between chunks, see rotate_left) keeps
@@ -358,7 +366,7 @@ __attribute__((amdgpu_waves_per_eu(reg_waves_per_eu<NMAX>()))) getrf_reg_kernel(
 // zero-padded scratch and never stored).
 template <int NMAX, bool SOLVE, bool VAR>
 __global__ void __launch_bounds__(kWave *kWavesPerBlock)
-__attribute__((amdgpu_waves_per_eu(reg_waves_per_eu<NMAX>()))) potrf_reg_kernel(MatSet s)
+__attribute__((amdgpu_waves_per_eu(potrf_waves_per_eu<NMAX, VAR>()))) potrf_reg_kernel(MatSet s)
 {
     constexpr int MPW = kWave / NMAX;
     constexpr int CH = reg_chunk<NMAX>();
@@ -398,7 +406,7 @@ __attribute__((amdgpu_waves_per_eu(reg_waves_per_eu<NMAX>()))) potrf_reg_kernel(
         buf[r] = 0.0;
 
         int info = 0;
-        double dg = 1.0;                         // this lane's diagonal entry l_rr
+        double rdg = 1.0;                        // 1 / l_rr of this lane's row
     #pragma unroll 1
         for (int jbase = 0; jbase < NMAX; jbase += CH) {
             const int kmax = NMAX - jbase;
@@ -423,7 +431,7 @@ __attribute__((amdgpu_waves_per_eu(reg_waves_per_eu<NMAX>()))) potrf_reg_kernel(
                         const double l = rc * a[jj];
                         if (r == j) {
                             a[jj] = ljj;
-                            dg = ljj;
+                            rdg = rc;
                         }
                         // the multipliers reach the other lanes through LDS (one
                         // write per lane, broadcast reads); at n = 64 that issues
@@ -457,11 +465,16 @@ __attribute__((amdgpu_waves_per_eu(reg_waves_per_eu<NMAX>()))) potrf_reg_kernel(
         if (it.live && r == 0) s.info[it.id] = info;
 
         if constexpr (SOLVE) {
-            // dpotrs('L'): L y = b (column sweep, division by l_kk), then L' x = y.
-            // The L' sweep runs column-oriented (row k of L handed to the lanes
-            // through LDS), so its sums associate in decreasing k where the
-            // reference dtrsm('L','L','T') accumulates a dot product in increasing
-            // k; the two agree to rounding.
+            // dpotrs('L'): L y = b (column sweep), then L' x = y.  Both sweeps
+            // scale by the reciprocal 1 / l_kk that the factorization formed
+            // for the multipliers (one multiply per right-hand side where the
+            // reference dtrsm divides: a division is a dozen dependent VALU
+            // instructions, issued by the whole wavefront for one lane, and
+            // was most of the solve's issue time).  The L' sweep runs
+            // column-oriented (row k of L handed to the lanes through LDS), so
+            // its sums associate in decreasing k where the reference
+            // dtrsm('L','L','T') accumulates a dot product in increasing k.
+            // Both agree with the reference to rounding.
             if (!it.live || info != 0) continue;
             const int nrhs = s.nrhs_arr ? s.nrhs_arr[it.id] : s.nrhs;
             const int ldb = s.ldb_arr ? s.ldb_arr[it.id] : s.ldb;
@@ -478,7 +491,7 @@ __attribute__((amdgpu_waves_per_eu(reg_waves_per_eu<NMAX>()))) potrf_reg_kernel(
                         if (r == k) {
     #pragma unroll
                             for (int q = 0; q < kRhsChunk; ++q) {
-                                if (y[q] != 0.0) y[q] = y[q] / a[k];
+                                y[q] = y[q] * rdg;
                                 if constexpr (NMAX != 64) xb[q] = y[q];
                             }
                         }
@@ -513,7 +526,7 @@ __attribute__((amdgpu_waves_per_eu(reg_waves_per_eu<NMAX>()))) potrf_reg_kernel(
                         for (int k = min(kb + kLRows - 1, n - 1); k >= kb; --k) {
                             if (r == k) {
     #pragma unroll
-                                for (int q = 0; q < kRhsChunk; ++q) y[q] = y[q] / dg;
+                                for (int q = 0; q < kRhsChunk; ++q) y[q] = y[q] * rdg;
                             }
                             double yk[kRhsChunk];
     #pragma unroll
@@ -533,7 +546,7 @@ __attribute__((amdgpu_waves_per_eu(reg_waves_per_eu<NMAX>()))) potrf_reg_kernel(
                         if (r == k) {
     #pragma unroll
                             for (int q = 0; q < kRhsChunk; ++q) {
-                                y[q] = y[q] / a[k];
+                                y[q] = y[q] * rdg;
                                 xb[q] = y[q];
                             }
     #pragma unroll

@@ -2,8 +2,10 @@
 
 Bar: info bit-exact; the Cholesky factor bit-exact (same update order and
 fused multiply-adds as the oracle); the upper triangle of A untouched; the
-forward solve bit-exact and the backward (L') solve to rounding, checked by
-the tester residual < 30 and the factor residual ||A - LL'||/(n||A||eps) < 30.
+solves to rounding (the fused n <= 64 posv scales by 1/l_kk where dtrsm
+divides, and its L' sweep associates in the other order), checked against the
+oracle's solution, by the tester residual < 30 and the factor residual
+||A - LL'||/(n||A||eps) < 30.
 """
 import numpy as np
 import pytest

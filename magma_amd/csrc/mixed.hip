@@ -71,6 +71,10 @@ constexpr int kUPad = CH + 4;   // staging row (halves): 72-byte stride, conflic
 
 __device__ __forceinline__ int pick(const int *arr, int fixed, int id) { return arr ? arr[id] : fixed; }
 
+typedef double __attribute__((address_space(1))) *GDoublePtr;
+typedef float __attribute__((address_space(1))) *GFloatPtr;
+typedef int __attribute__((address_space(1))) *GIntPtr;
+
 struct alignas(16) FSlot {
     unsigned key;     // pivot_key of the wavefront's best row
     int cur;
@@ -1008,13 +1012,21 @@ __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(OCC))) 
             }
             continue;
         }
-        float *F = g.F ? g.F[id] : g.Wslot + g.slot_floats * blockIdx.x;
-        const double *A = g.A[id];
+        // Every matrix, vector and factor pointer is read through a global
+        // (address space 1) pointer type and converted back, so the compiler
+        // emits global rather than flat loads and stores for them: a flat
+        // access also counts against the LDS wait counter, and every LDS wait
+        // of the sweeps, panels and residual would drain the global loads in
+        // flight.
+        float *F = g.F ? (float *)((GFloatPtr *)g.F)[id] : (float *)(GFloatPtr)(g.Wslot + g.slot_floats * blockIdx.x);
+        const double *A = (const double *)((GDoublePtr *)g.A)[id];
+        double *Bm = (double *)((GDoublePtr *)g.B)[id];
+        double *Xm = (double *)((GDoublePtr *)g.X)[id];
         int status = 0;
         if (g.reuse && g.reuse[id]) {
             M::load_factor_meta(sm, F, n, chol);
         } else {
-            status = M::factor(sm, A, lda, n, F, (!chol && g.ipiv) ? g.ipiv[id] : nullptr, chol);
+            status = M::factor(sm, A, lda, n, F, (!chol && g.ipiv) ? (int *)((GIntPtr *)g.ipiv)[id] : nullptr, chol);
             __syncthreads();
         }
         int iters = 0;
@@ -1023,7 +1035,7 @@ __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(OCC))) 
         for (int g0 = 0; g0 < nrhs && status == 0; g0 += RG) {
             const int nr = min(RG, nrhs - g0);
             double be = 0.0;
-            const int r = M::refine(sm, A, lda, g.B[id] + (size_t)g0 * ldb, ldb, g.X[id] + (size_t)g0 * ldx, ldx,
+            const int r = M::refine(sm, A, lda, Bm + (size_t)g0 * ldb, ldb, Xm + (size_t)g0 * ldx, ldx,
                                     F, n, nr, chol, anrm, be);
             if (r < 0) status = r;
             else iters = max(iters, r);

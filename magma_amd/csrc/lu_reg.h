@@ -85,6 +85,17 @@ __device__ __forceinline__ void rotate_left(double (&a)[NMAX])
     for (int k = 0; k < CH; ++k) a[NMAX - CH + k] = t[k];
 }
 
+// p[id] read as a global (address space 1) pointer and converted back: the
+// compiler then emits global rather than flat accesses through it.  A flat
+// load also counts against the LDS wait counter, so the first LDS hand-off of
+// the column loop would wait for every entry of the row, where a global load
+// is waited for only when its register is first used.
+__device__ __forceinline__ double *global_ptr(double *const *p, int id)
+{
+    typedef double __attribute__((address_space(1))) *GDoublePtr;
+    return (double *)((GDoublePtr const *)p)[id];
+}
+
 // Per-item sizes: scalars for fixed-size batches (VAR = false), per-matrix
 // device arrays for variable-size batches.
 template <bool VAR>
@@ -255,7 +266,7 @@ __attribute__((amdgpu_waves_per_eu(reg_waves_per_eu<NMAX>()))) getrf_reg_kernel(
         const Item<VAR> it(s, (blk * kWavesPerBlock + w) * MPW + g, cnt, idx);
         const int m = it.m, n = it.n, lda = it.lda;
         const bool rowok = it.live && r < m;
-        double *A = it.live ? s.A[it.id] : nullptr;
+        double *A = it.live ? global_ptr(s.A, it.id) : nullptr;
         const int mn = min(m, n);
 
         double a[NMAX];
@@ -351,7 +362,7 @@ __attribute__((amdgpu_waves_per_eu(reg_waves_per_eu<NMAX>()))) getrf_reg_kernel(
             if (!it.live || info != 0) continue;
             const int nrhs = s.nrhs_arr ? s.nrhs_arr[it.id] : s.nrhs;
             const int ldb = s.ldb_arr ? s.ldb_arr[it.id] : s.ldb;
-            double *Bm = s.B[it.id];
+            double *Bm = global_ptr(s.B, it.id);
             // one right-hand side at a time when the system has one (a group
             // of four otherwise would carry three columns of zeros)
             if (nrhs == 1) getrs_reg<NMAX, 1>(a, pos, r, n, Bm, ldb, 0, 1, xb);
@@ -391,7 +402,7 @@ __attribute__((amdgpu_waves_per_eu(potrf_waves_per_eu<NMAX, VAR>()))) potrf_reg_
         const Item<VAR> it(s, (blk * kWavesPerBlock + w) * MPW + g, cnt, idx);
         const int n = it.n, lda = it.lda;
         const bool rowok = it.live && r < n;
-        double *A = it.live ? s.A[it.id] : nullptr;
+        double *A = it.live ? global_ptr(s.A, it.id) : nullptr;
 
         double a[NMAX];
     #pragma unroll
@@ -478,7 +489,7 @@ __attribute__((amdgpu_waves_per_eu(potrf_waves_per_eu<NMAX, VAR>()))) potrf_reg_
             if (!it.live || info != 0) continue;
             const int nrhs = s.nrhs_arr ? s.nrhs_arr[it.id] : s.nrhs;
             const int ldb = s.ldb_arr ? s.ldb_arr[it.id] : s.ldb;
-            double *Bm = s.B[it.id];
+            double *Bm = global_ptr(s.B, it.id);
             for (int c0 = 0; c0 < nrhs; c0 += kRhsChunk) {
                 double y[kRhsChunk];
     #pragma unroll

@@ -17,6 +17,8 @@
 // A(l%16, l/16) and B(l/16, l%16) of the 16x4 / 4x16 blocks and receives
 // D(l/16 + 4*i, l%16), i = 0..3, of the 16x16 result (FP64 interleaves the
 // result rows, unlike the FP16/FP32 variants' 4*(l/16) + i).
+#include <type_traits>
+
 #include "kernels.h"
 #include "workqueue.h"
 
@@ -28,6 +30,19 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 constexpr int kWaves = 4;
 
 __device__ __forceinline__ int pick(const int *arr, int fixed, int id) { return arr ? arr[id] : fixed; }
+
+// p[id] read as a global (address space 1) pointer and converted back, so
+// the compiler emits global rather than flat loads and stores through it.  A
+// flat load also counts against the LDS wait counter: the next k block's
+// operand loads, issued ahead of the current block's MFMAs, would be waited
+// for at the first LDS read of the current block.
+template <class T>
+__device__ __forceinline__ T *global_ptr(T *const *p, int id)
+{
+    typedef typename std::remove_const<T>::type U;
+    typedef U __attribute__((address_space(1))) *G;
+    return (T *)((G const *)p)[id];
+}
 
 template <bool TA, bool TB, int BT>
 __global__ void __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(BT == 1 ? 4 : (BT == 2 ? 3 : 2)))) gemm_mfma_kernel(GemmSet g, const int *prefix, int fixed_tiles)
@@ -61,9 +76,9 @@ __global__ void __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_e
         const int mt = (m + TS - 1) / TS;
         const int row0 = (local % mt) * TS;
         const int col0 = (local / mt) * TS;
-        const double *A = g.A[id];
-        const double *B = g.B[id];
-        double *C = g.C[id];
+        const double *A = global_ptr(g.A, id);
+        const double *B = global_ptr(g.B, id);
+        double *C = global_ptr(g.C, id);
 
         d4 acc[BT][BT];
 #pragma unroll
@@ -210,9 +225,9 @@ gemm_lds_kernel(GemmSet g, const int *prefix, int fixed_tiles)
         const int mt = (m + kLT - 1) / kLT;
         const int row0 = (local % mt) * kLT;
         const int col0 = (local / mt) * kLT;
-        const double *A = g.A[id];
-        const double *B = g.B[id];
-        double *C = g.C[id];
+        const double *A = global_ptr(g.A, id);
+        const double *B = global_ptr(g.B, id);
+        double *C = global_ptr(g.C, id);
 
         d4 acc[2][2];
 #pragma unroll

@@ -5,6 +5,14 @@
 
 namespace magma_amd {
 
+// Pointers in the global address space.  A device pointer read from a pointer
+// array through one of these and converted back to a generic pointer lets the
+// compiler emit global rather than flat accesses through it (a flat access
+// also counts against the LDS wait counter, so LDS waits would drain it).
+typedef double __attribute__((address_space(1))) *GDoublePtr;
+typedef float __attribute__((address_space(1))) *GFloatPtr;
+typedef int __attribute__((address_space(1))) *GIntPtr;
+
 // Describes a set of matrices for one launch.  Sizes either fixed (arrays
 // null) or per matrix (device arrays indexed by matrix id).  `index`, when not
 // null, lists the matrix ids this launch processes (a bucket of a vbatched

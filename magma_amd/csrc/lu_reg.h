@@ -92,7 +92,6 @@ __device__ __forceinline__ void rotate_left(double (&a)[NMAX])
 // is waited for only when its register is first used.
 __device__ __forceinline__ double *global_ptr(double *const *p, int id)
 {
-    typedef double __attribute__((address_space(1))) *GDoublePtr;
     return (double *)((GDoublePtr const *)p)[id];
 }
 

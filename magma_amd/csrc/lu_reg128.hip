@@ -206,8 +206,8 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(W
     for (int item = blockIdx.x; item < cnt; item += gridDim.x) {
         const int id = idx ? idx[item] : item;
         const int m = pick(s.m_arr, s.m, id), n = pick(s.n_arr, s.n, id), lda = pick(s.lda_arr, s.lda, id);
-        double *A = s.A[id];
-        int *ipiv = s.ipiv ? s.ipiv[id] : nullptr;
+        double *A = (double *)((GDoublePtr *)s.A)[id];
+        int *ipiv = s.ipiv ? (int *)((GIntPtr *)s.ipiv)[id] : nullptr;
         const int mn = min(m, n);
         const bool ok0 = lane < m, ok1 = lane + 64 < m;
 

@@ -71,10 +71,6 @@ constexpr int kUPad = CH + 4;   // staging row (halves): 72-byte stride, conflic
 
 __device__ __forceinline__ int pick(const int *arr, int fixed, int id) { return arr ? arr[id] : fixed; }
 
-typedef double __attribute__((address_space(1))) *GDoublePtr;
-typedef float __attribute__((address_space(1))) *GFloatPtr;
-typedef int __attribute__((address_space(1))) *GIntPtr;
-
 struct alignas(16) FSlot {
     unsigned key;     // pivot_key of the wavefront's best row
     int cur;

@@ -502,17 +502,14 @@ __attribute__((amdgpu_waves_per_eu(potrf_waves_per_eu<NMAX, VAR>()))) potrf_reg_
     #pragma unroll
                             for (int q = 0; q < kRhsChunk; ++q) {
                                 y[q] = y[q] * rdg;
-                                if constexpr (NMAX != 64) xb[q] = y[q];
+                                xb[q] = y[q];
                             }
                         }
-                        if constexpr (NMAX == 64) {
+                        // y_k to the other lanes through LDS (at n = 64 as well:
+                        // readlane pairs measured 3-4% slower on 20,000 posv)
+                        wave_lds_sync();
     #pragma unroll
-                            for (int q = 0; q < kRhsChunk; ++q) yk[q] = readlane_f64(y[q], k);
-                        } else {
-                            wave_lds_sync();
-    #pragma unroll
-                            for (int q = 0; q < kRhsChunk; ++q) yk[q] = xb[q];
-                        }
+                        for (int q = 0; q < kRhsChunk; ++q) yk[q] = xb[q];
                         if (rowok && r > k) {
     #pragma unroll
                             for (int q = 0; q < kRhsChunk; ++q) y[q] = fma(-yk[q], a[k], y[q]);
@@ -523,7 +520,7 @@ __attribute__((amdgpu_waves_per_eu(potrf_waves_per_eu<NMAX, VAR>()))) potrf_reg_
                 if constexpr (NMAX == 64) {
                     // the same sweep, rolled: rows of L go to LDS kLRows at a
                     // time (every lane of the group writes its own row at once)
-                    // and x_k reaches the other lanes by readlane
+                    // and x_k reaches the other lanes through LDS
     #pragma unroll 1
                     for (int kb = ((n - 1) / kLRows) * kLRows; kb >= 0; kb -= kLRows) {
                         if (rowok && r >= kb && r < kb + kLRows) {
@@ -536,11 +533,15 @@ __attribute__((amdgpu_waves_per_eu(potrf_waves_per_eu<NMAX, VAR>()))) potrf_reg_
                         for (int k = min(kb + kLRows - 1, n - 1); k >= kb; --k) {
                             if (r == k) {
     #pragma unroll
-                                for (int q = 0; q < kRhsChunk; ++q) y[q] = y[q] * rdg;
+                                for (int q = 0; q < kRhsChunk; ++q) {
+                                    y[q] = y[q] * rdg;
+                                    xb[q] = y[q];
+                                }
                             }
+                            wave_lds_sync();
                             double yk[kRhsChunk];
     #pragma unroll
-                            for (int q = 0; q < kRhsChunk; ++q) yk[q] = readlane_f64(y[q], k);
+                            for (int q = 0; q < kRhsChunk; ++q) yk[q] = xb[q];
                             if (rowok && r < k) {
                                 const double lk = lrows[w][k - kb][r];
     #pragma unroll
